@@ -1,0 +1,246 @@
+/* fluxion_hip.h - C ABI of the MI355X (gfx950) force-directed placement
+ * inner loop.
+ *
+ * This header is the drop-in boundary that replaces the reference's GPU
+ * operator layer (HAPR_fluxion_v4/src/gpu/{context,dispatch}.rs) and the CPU
+ * loops it was meant to accelerate. Plain C types only: host arrays are
+ * borrowed for the duration of a call, device memory is owned by the handles
+ * below, and every function returns 0 on success or a negative FX_E* code;
+ * fx_last_error() then describes the failure (thread-local). Nothing falls
+ * back to the CPU: without a usable GPU, fx_ctx_create fails and the caller
+ * reports the error (GpuContext::new returning None, context.rs:17-19, but
+ * with no CPU fallback behind it).
+ */
+#ifndef FLUXION_HIP_H_
+#define FLUXION_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FX_OK 0
+#define FX_E_INVALID -1   /* bad argument or shape */
+#define FX_E_DEVICE -2    /* HIP runtime error (no device, launch failure) */
+#define FX_E_NOMEM -3     /* device allocation failed or over the HBM budget */
+#define FX_E_STATE -4     /* call not valid in the handle's current state */
+#define FX_E_BUSY -5      /* server queue full / request not finished */
+
+typedef struct fx_ctx fx_ctx;
+typedef struct fx_job fx_job;
+typedef struct fx_server fx_server;
+
+/* Placement parameters. The first five fields are PlacerConfig
+ * (HAPR_fluxion_v4/src/algorithm/placer.rs:13-31, defaults 50, 1.0, 0.5,
+ * 0.9, 0.1; YAML keys of config/default.yaml:22-27). The rest configure the
+ * flat global loop: Barnes-Hut opening angle and distance clamp
+ * (neon_old_codebase/src/cpp/BarnesHutTree.cpp:145-168), and the DCT
+ * density force (neon_old_codebase/src/python/fluxion/force_electrostatic.py:
+ * 36-37, weight and target density; density_grid 0 disables it). */
+typedef struct fx_params {
+  int32_t force_iterations;
+  float attract_weight;
+  float repel_weight;
+  float damping;
+  float dt;
+  float theta;
+  float min_dist;
+  float density_weight;
+  float target_density;
+  int32_t density_grid;
+  int32_t dct_passes;  /* split-BF16 MFMA passes: 3 (hi*hi, hi*lo, lo*hi) or 4 (+lo*lo) */
+  int32_t flags;       /* FX_FLAG_* */
+} fx_params;
+
+#define FX_FLAG_NO_SPRINGS 1
+#define FX_FLAG_NO_REPULSION 2
+#define FX_FLAG_NO_DENSITY 4
+
+/* Fills PlacerConfig::default() and the flat-loop defaults
+ * (theta 0.5, min_dist 0.1, density weight 1.0, target 0.8, grid 0). */
+void fx_params_default(fx_params* p);
+
+/* A flat netlist: Circuit.net_gate_indices (circuit.rs:62) as CSR and the
+ * die of Circuit.chip_width/chip_height (circuit.rs:75-76). Gates are unit
+ * size. */
+typedef struct fx_netlist {
+  int32_t num_gates;
+  int32_t num_nets;
+  const int32_t* net_ptr;  /* [num_nets+1] */
+  const int32_t* net_pins; /* [net_ptr[num_nets]] gate ids */
+  float chip_width;
+  float chip_height;
+} fx_netlist;
+
+/* ---- context (GpuContext, HAPR_fluxion_v4/src/gpu/context.rs:17-70) ---- */
+int fx_ctx_create(int device, fx_ctx** out);
+void fx_ctx_destroy(fx_ctx* ctx);
+const char* fx_last_error(void);
+/* Adapter name (GpuContext.adapter_name, context.rs:14). */
+const char* fx_ctx_device_name(const fx_ctx* ctx);
+/* Bytes of device memory currently held by the context's jobs. */
+int64_t fx_ctx_bytes_in_use(const fx_ctx* ctx);
+
+/* ---- batched leaf placement ----
+ * The leaf loop of hapr_placement (placer.rs:66-69) calling
+ * place_gates_in_leaf (placer.rs:210-287) for every leaf, in ONE persistent
+ * kernel launch: per leaf the initial positions of placer.rs:226-227,
+ * adjacency springs over in-leaf neighbours (242-255), exact O(n^2)
+ * repulsion (257-272) and the damped, clamped Euler step (274-280), for
+ * force_iterations iterations. Writes x[g], y[g] for every leaf gate g
+ * (Placement::set_position, 284-286). Leaves of up to 1024 gates. */
+typedef struct fx_leaf_batch {
+  int32_t num_leaves;
+  const int32_t* leaf_ptr;   /* [num_leaves+1] into leaf_gates */
+  const int32_t* leaf_gates; /* ClusterNode.gate_indices (hierarchy.rs:15) */
+  const float* region;       /* [4*num_leaves] center_x, center_y, width, height (hierarchy.rs:20-23) */
+  int32_t num_gates;
+  const int32_t* adj_ptr;    /* Circuit.adjacency (circuit.rs:68) as CSR */
+  const int32_t* adj;
+} fx_leaf_batch;
+
+#define FX_LEAF_EXACT 0 /* IEEE div/sqrt in the reference's order: bit-identical to placer.rs in FP32 */
+#define FX_LEAF_FAST 1  /* rsqrt/rcp with FMA: within 1e-4 of the exact forces per iteration */
+
+int fx_place_leaves(fx_ctx* ctx, const fx_leaf_batch* batch, const fx_params* params,
+                    int32_t mode, float* x, float* y);
+/* Same with device-resident state for teacher-forced checks: starts from the
+ * given positions/velocities (no initialisation) when init==0, runs
+ * `iterations` and returns the last iteration's forces when fx/fy are set. */
+int fx_place_leaves_state(fx_ctx* ctx, const fx_leaf_batch* batch, const fx_params* params,
+                          int32_t mode, int32_t init, int32_t iterations, float* x, float* y,
+                          float* vx, float* vy, float* fx, float* fy);
+/* Milliseconds of the last fx_place_leaves* kernel (HIP events). */
+double fx_leaves_last_kernel_ms(const fx_ctx* ctx);
+
+/* ---- flat global placement job (device-resident state) ----
+ * The dispatch_forces operator (HAPR_fluxion_v4/src/gpu/dispatch.rs:18-106)
+ * reworked so that the netlist, positions, velocities, Barnes-Hut tree and
+ * density grids stay in HBM across calls. One iteration = net springs (CSR
+ * segmented reductions) + Barnes-Hut repulsion + DCT Poisson density force,
+ * then the damped Euler step of placer.rs:274-280 clamped to the die. */
+int fx_job_create(fx_ctx* ctx, const fx_netlist* net, const fx_params* params, fx_job** out);
+void fx_job_destroy(fx_job* job);
+int fx_job_set_positions(fx_job* job, const float* x, const float* y, int64_t n);
+int fx_job_get_positions(fx_job* job, float* x, float* y, int64_t n);
+int fx_job_set_velocities(fx_job* job, const float* vx, const float* vy, int64_t n);
+int fx_job_get_velocities(fx_job* job, float* vx, float* vy, int64_t n);
+int fx_job_run(fx_job* job, int32_t iterations);
+/* Force of each component at the current positions (no state change):
+ * component 1 springs, 2 Barnes-Hut, 4 density; any OR of them. */
+int fx_job_forces(fx_job* job, int32_t components, float* fx, float* fy);
+/* Density diagnostics of the current positions (any pointer may be NULL):
+ * density, potential phi, field Ex, Ey, each density_grid^2 row-major. */
+int fx_job_density_fields(fx_job* job, float* rho, float* phi, float* ex, float* ey);
+/* Barnes-Hut statistics of the last tree: [0] nodes, [1] merge depth,
+ * [2] accepted interactions, [3] opened nodes. */
+int fx_job_bh_stats(fx_job* job, int64_t* stats);
+/* Enqueue on the job's stream, do not wait. */
+int fx_job_run_async(fx_job* job, int32_t iterations);
+int fx_job_sync(fx_job* job);
+/* Capture one iteration in a hipGraph and replay it (bit-identical to the
+ * eager path); iterations run as graph launches from then on. */
+int fx_job_use_graph(fx_job* job, int32_t enable);
+
+/* Per-phase HIP-event timing of eager iterations on the job's stream (no
+ * counterpart in the reference; used by the benchmark). phase_mask bit k
+ * times phase k: 0 net centroids, 1 Barnes-Hut keys, 2 key sort, 3 tree
+ * build, 4 tree walk, 5 DCT density, 6 force combine + integrator. While a
+ * mask is set, iterations run eagerly (no graph). read_timing returns the
+ * summed milliseconds and launch counts per phase ([7] each) and resets. */
+int fx_job_set_timing(fx_job* job, int32_t phase_mask);
+int fx_job_read_timing(fx_job* job, double* total_ms, int64_t* count);
+
+/* dispatch_congestion (dispatch.rs:109-180) / compute_congestion_cpu
+ * (congestion.rs:11-62): per-net bounding box, +1 on every covered cell of a
+ * cols x rows grid over the die; h and v grids are identical
+ * (congestion.rs:42-60), so one int32 count grid is returned (bit-exact).
+ * Cell of a point: RoutingGrid::cell_at (grid.rs:58-62). */
+int fx_congestion(fx_job* job, int32_t cols, int32_t rows, int32_t* counts);
+/* Placement::compute_hpwl (placement.rs:48-79), reduced in FP64. */
+int fx_hpwl(fx_job* job, double* hpwl);
+/* RoutingGrid::total_overflow / max_overflow (grid.rs:70-91) of the
+ * congestion grid with h = v = count, capacity (cw+ch)/2, in FP64. */
+int fx_overflow(fx_job* job, int32_t cols, int32_t rows, double* total, double* max);
+/* Density overflow of benchmark_runner.py:264-285 (bins x bins grid,
+ * target 0.8): [0] max utilisation, [1] overflow. */
+int fx_density_overflow(fx_job* job, int32_t bins, double* max_density, double* overflow);
+
+/* One-shot congestion on host arrays (dispatch_congestion's signature:
+ * grid, placement, circuit). */
+int fx_dispatch_congestion(fx_ctx* ctx, const fx_netlist* net, const float* x,
+                           const float* y, int32_t cols, int32_t rows, int32_t* counts);
+
+/* ---- power iteration (compute_fiedler_vector, partitioner.rs:106-185) ----
+ * Laplacian of the subgraph induced by gates[0..n) over the global adjacency
+ * CSR (partitioner.rs:116-137), FP32 SpMV in the reference's summation order,
+ * mean deflation and L2 normalisation reduced in FP64 on the GPU. v0: the
+ * initial vector (partitioner.rs:145-147); it is normalised first.
+ * rayleigh (optional): v^T L v of the result in FP64. */
+int fx_fiedler(fx_ctx* ctx, int32_t n, const int32_t* gates, int32_t num_gates,
+               const int32_t* adj_ptr, const int32_t* adj, const float* v0,
+               int32_t iterations, float* v_out, double* rayleigh);
+/* Milliseconds of the last fx_fiedler iteration loop (HIP events). */
+double fx_fiedler_last_ms(const fx_ctx* ctx);
+
+/* ---- sort (split_by_fiedler's stable sort_by(partial_cmp),
+ * partitioner.rs:196-202; Morton ordering of the Barnes-Hut build) ----
+ * Stable LSD radix sort on the GPU; perm receives the source index of each
+ * sorted position. f32 keys compare as partial_cmp does (-0.0 == +0.0);
+ * NaN keys are rejected with FX_E_INVALID. */
+int fx_sort_u64(fx_ctx* ctx, int64_t n, const uint64_t* keys, uint32_t* perm);
+int fx_sort_f32(fx_ctx* ctx, int64_t n, const float* keys, uint32_t* perm);
+/* split_by_fiedler (partitioner.rs:189-213): stable sort of gates by value,
+ * balanced split; writes the left half then the right half into out and
+ * returns the left size through *left. */
+int fx_split_by_fiedler(fx_ctx* ctx, int32_t n, const float* fiedler, const int32_t* gates,
+                        float balance_factor, int32_t* out, int32_t* left);
+
+/* ---- placement serving (continuous batching) ----
+ * A server thread admits placement requests as slots and HBM budget free up,
+ * keeps each request's state on the device, runs active requests' iterations
+ * concurrently (one stream per slot, graph-captured iterations) and retires
+ * them independently. Each result equals the same request run alone. */
+typedef struct fx_server_config {
+  int32_t max_active;     /* concurrent slots */
+  int64_t hbm_budget;     /* bytes the server may hold on the device */
+  int32_t use_graphs;     /* capture each request's iteration in a hipGraph */
+} fx_server_config;
+
+int fx_server_create(fx_ctx* ctx, const fx_server_config* cfg, fx_server** out);
+/* Copies the netlist and initial positions; returns a request id. */
+int fx_server_submit(fx_server* s, const fx_netlist* net, const fx_params* params,
+                     const float* x0, const float* y0, int32_t iterations, int64_t* id);
+/* status: 0 queued, 1 running, 2 done, 3 failed. When done, copies the final
+ * positions (if x/y set) and the FP64 HPWL, and reports timestamps (seconds
+ * on the server clock) of submit, start and finish. */
+int fx_server_poll(fx_server* s, int64_t id, int32_t* status, float* x, float* y,
+                   double* hpwl, double* t_submit, double* t_start, double* t_finish);
+/* Releases a finished request's host copy. */
+int fx_server_release(fx_server* s, int64_t id);
+/* Blocks until every submitted request is done or failed. */
+int fx_server_drain(fx_server* s);
+double fx_server_clock(const fx_server* s);
+void fx_server_destroy(fx_server* s);
+
+/* ---- host utilities (no GPU) ---- */
+/* Circuit::build_adjacency (circuit.rs:154-171) as CSR; adj NULL: count only. */
+int64_t fx_build_adjacency(int32_t num_gates, int32_t num_nets, const int32_t* net_ptr,
+                           const int32_t* net_pins, int32_t* adj_ptr, int32_t* adj,
+                           int64_t capacity);
+/* Seeded synthetic netlists (distributions stated in netgen.cpp). With NULL
+ * output buffers only the pin count is returned. */
+int64_t fx_netgen_flat(int32_t num_gates, int32_t num_nets, uint64_t seed, float die_w,
+                       float die_h, int32_t* net_ptr, int32_t* net_pins,
+                       int64_t pin_capacity, float* x, float* y);
+int64_t fx_netgen_leaves(int32_t num_leaves, uint64_t seed, int32_t min_n, int32_t max_n,
+                         float region, int32_t* leaf_ptr, float* leaf_region,
+                         int32_t* net_ptr, int32_t* net_pins, int64_t* counts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUXION_HIP_H_ */

@@ -1,0 +1,443 @@
+// Barnes-Hut repulsion (neon_old_codebase/src/cpp/BarnesHutTree.cpp:16-168)
+// rebuilt for the GPU as a linear quadtree over sorted cell keys.
+//
+// The reference inserts particles one by one into a pointer quadtree. The
+// tree it builds depends only on the points: a cell with one particle is a
+// leaf, a cell with two or more is split at its midpoint (px <= mx goes W,
+// py <= my goes S; BarnesHutTree.cpp:107-123), and particles still sharing a
+// cell of size < 1e-8 are merged into one leaf (64-72). Here:
+//   1. bbox reduction -> root cell (24-46) and the merge depth D, in FP64;
+//   2. per particle, its quadrant at each of D levels by descending the FP64
+//      midpoints exactly as the reference does -> a 2D-bit key whose digit
+//      order (NW=0, NE=1, SW=2, SE=3) is the reference's child order;
+//   3. stable radix sort of the keys (equal keys keep index order, so a
+//      merged leaf's first particle is its lowest index, as in insertion);
+//   4. L[i] = common levels of sorted keys i, i+1. The cells of the tree are
+//      the runs of keys sharing a prefix; listing, for each sorted position
+//      i, the cells that start at i (depths L[i-1]+1..L[i], then its leaf)
+//      enumerates the tree in DFS pre-order — the reference's recursion
+//      order (computeForceRec, 145-168). A scan of the per-position counts
+//      gives node ids; a cell's last particle is the first q >= i with
+//      L[q] < depth (found through two levels of block minima), which gives
+//      its particle count, its fixed-point coordinate sums (exclusive scans
+//      of 64-bit fixed-point positions — exact, so order-free) and its skip
+//      pointer (the first node after its subtree);
+//   5. per particle, a stackless walk of the pre-order array: skip the leaf
+//      holding the particle (149), accept a leaf or a cell with
+//      size/dist < theta (157) and add F = k q Q / d^2 along (p - c)/d with
+//      d >= min_dist (151-160), else descend to the next node (first child).
+// The walk visits accepted cells in the reference's order, so each
+// particle's force is summed in the same order as the CPU path of record.
+
+#include <cmath>
+
+#include "job.h"
+
+namespace fx {
+
+int64_t scan_tmp_elems(int64_t n);
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct I64x2 {
+  long long a, b;
+};
+
+__global__ __launch_bounds__(kBlock) void bh_bbox_partial(int n, const float* __restrict__ x,
+                                                          const float* __restrict__ y,
+                                                          float4* __restrict__ part) {
+  float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    mnx = fminf(mnx, x[i]);
+    mxx = fmaxf(mxx, x[i]);
+    mny = fminf(mny, y[i]);
+    mxy = fmaxf(mxy, y[i]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    mnx = fminf(mnx, __shfl_down(mnx, off, 64));
+    mxx = fmaxf(mxx, __shfl_down(mxx, off, 64));
+    mny = fminf(mny, __shfl_down(mny, off, 64));
+    mxy = fmaxf(mxy, __shfl_down(mxy, off, 64));
+  }
+  __shared__ float4 sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = make_float4(mnx, mxx, mny, mxy);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float4 r = sh[0];
+    for (int w = 1; w < kBlock / 64; ++w) {
+      r.x = fminf(r.x, sh[w].x);
+      r.y = fmaxf(r.y, sh[w].y);
+      r.z = fminf(r.z, sh[w].z);
+      r.w = fmaxf(r.w, sh[w].w);
+    }
+    part[blockIdx.x] = r;
+  }
+}
+
+__device__ __forceinline__ int bitlen_u64(unsigned long long v) {
+  return v ? 64 - __clzll((long long)v) : 0;
+}
+
+// Root cell (BarnesHutTree.cpp:24-46), merge depth and fixed-point scale.
+__global__ void bh_geometry(const float4* __restrict__ part, int nparts, int n,
+                            BhGeometry* __restrict__ g) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float4 r = part[0];
+  for (int k = 1; k < nparts; ++k) {
+    r.x = fminf(r.x, part[k].x);
+    r.y = fmaxf(r.y, part[k].y);
+    r.z = fminf(r.z, part[k].z);
+    r.w = fmaxf(r.w, part[k].w);
+  }
+  const double min_x = (double)r.x - 1.0, max_x = (double)r.y + 1.0;
+  const double min_y = (double)r.z - 1.0, max_y = (double)r.w + 1.0;
+  const double size = fmax(max_x - min_x, max_y - min_y);
+  const double cx = (min_x + max_x) / 2.0, cy = (min_y + max_y) / 2.0;
+  const double root0 = cx - size / 2, root1 = cy - size / 2;
+  const double root2 = cx + size / 2, root3 = cy + size / 2;
+  double xl = root0, yl = root1, xh = root2, yh = root3;
+  int d = 0;
+  while (d < 32 && fmax(xh - xl, yh - yl) >= 1e-8) {
+    xh = (xl + xh) / 2;
+    yh = (yl + yh) / 2;
+    ++d;
+  }
+  const double s_root = fmax(root2 - root0, root3 - root1);
+  const int n_bits = bitlen_u64((unsigned long long)(n > 0 ? n : 1));
+  const unsigned long long s_int = (unsigned long long)ceil(s_root);
+  const int s_bits = bitlen_u64(s_int > 0 ? s_int : 1ull);
+  int sb = 62 - n_bits - s_bits;
+  sb = sb < 0 ? 0 : (sb > 40 ? 40 : sb);
+  g->root[0] = root0;
+  g->root[1] = root1;
+  g->root[2] = root2;
+  g->root[3] = root3;
+  g->depth = d;
+  g->scale_bits = sb;
+  g->scale = ldexp(1.0, sb);
+  g->num_nodes = 0;
+  g->overflow = 0;
+}
+
+// Quadrant path of each particle (insertIntoChild, BarnesHutTree.cpp:100-134).
+__global__ void bh_keys(int n, const float* __restrict__ x, const float* __restrict__ y,
+                        const BhGeometry* __restrict__ g, uint64_t* __restrict__ keys,
+                        uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double px = (double)x[i], py = (double)y[i];
+  double xl = g->root[0], yl = g->root[1], xh = g->root[2], yh = g->root[3];
+  const int D = g->depth;
+  uint64_t key = 0;
+  for (int d = 0; d < D; ++d) {
+    const double mx = (xl + xh) / 2, my = (yl + yh) / 2;
+    uint64_t quad;
+    if (px <= mx) {
+      xh = mx;
+      if (py <= my) { quad = 2; yh = my; }
+      else { quad = 0; yl = my; }
+    } else {
+      xl = mx;
+      if (py <= my) { quad = 3; yh = my; }
+      else { quad = 1; yl = my; }
+    }
+    key |= quad << (62 - 2 * d);
+  }
+  keys[i] = key;
+  vals[i] = (uint32_t)i;
+}
+
+__device__ __forceinline__ int lcp_levels(const uint64_t* __restrict__ keys, int n, int a, int D) {
+  if (a < 0 || a >= n - 1) return -1;
+  const uint64_t x = keys[a] ^ keys[a + 1];
+  const int l = x ? (__clzll((long long)x) >> 1) : 32;
+  return l < D ? l : D;
+}
+
+// L[i], per-position node counts, fixed-point positions in sorted order and
+// the block minima of L (blocks of 256, super-blocks of 256 blocks).
+__global__ __launch_bounds__(kBlock) void bh_lcp_count(
+    int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    const float* __restrict__ x, const float* __restrict__ y, const BhGeometry* __restrict__ g,
+    int8_t* __restrict__ L, int* __restrict__ bmin1, int* __restrict__ bmin2,
+    int* __restrict__ owned, I64x2* __restrict__ fix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int D = g->depth;
+  int li = 127;
+  if (i < n) {
+    li = lcp_levels(keys, n, i, D);
+    const int lp = lcp_levels(keys, n, i - 1, D);
+    L[i] = (int8_t)li;
+    const int internal = max(0, min(li, D - 1) - lp);
+    owned[i] = (lp >= D) ? 0 : internal + 1;
+    const uint32_t p = vals[i];
+    const double sc = g->scale;
+    fix[i].a = llrint(((double)x[p] - g->root[0]) * sc);
+    fix[i].b = llrint(((double)y[p] - g->root[1]) * sc);
+  }
+  int m = li;
+  for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_down(m, off, 64));
+  __shared__ int sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) m = min(m, sh[w]);
+    bmin1[blockIdx.x] = m;
+    atomicMin(&bmin2[blockIdx.x >> 8], m);
+  }
+}
+
+// Last sorted position of the cell of depth d that starts at position i:
+// the first q >= i with L[q] < d (L[n-1] = -1 ends every search).
+__device__ int find_end(const int8_t* __restrict__ L, const int* __restrict__ b1,
+                        const int* __restrict__ b2, int n, int nb1, int nb2, int i, int d) {
+  int q = i;
+  const int blk_end = min(n, ((q >> 8) + 1) << 8);
+  for (; q < blk_end; ++q)
+    if (L[q] < d) return q;
+  int b = q >> 8;
+  const int sb_end = min(nb1, ((b >> 8) + 1) << 8);
+  for (; b < sb_end; ++b)
+    if (b1[b] < d) break;
+  if (b == sb_end) {
+    int s = b >> 8;
+    for (; s < nb2; ++s)
+      if (b2[s] < d) break;
+    b = s << 8;
+    for (; b < nb1; ++b)
+      if (b1[b] < d) break;
+  }
+  for (q = b << 8; q < n; ++q)
+    if (L[q] < d) return q;
+  return n - 1;
+}
+
+__global__ void bh_emit(int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                        const int8_t* __restrict__ L, const int* __restrict__ bmin1,
+                        const int* __restrict__ bmin2, int nb1, int nb2,
+                        const int* __restrict__ offset, const I64x2* __restrict__ fixpre,
+                        BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int D = g->depth;
+  const int M = offset[n];
+  if (i == 0) {
+    g->num_nodes = M;
+    if ((int64_t)M > cap) g->overflow = 1;
+  }
+  const int lp = i > 0 ? (int)L[i - 1] : -1;
+  const int li = L[i];
+  if (lp >= D) return;  // continuation of a merged leaf
+  const int internal = max(0, min(li, D - 1) - lp);
+  const int id0 = offset[i];
+  const uint64_t key = keys[i];
+  const double r0 = g->root[0], r1 = g->root[1], scale = g->scale;
+  // Walk down the key once, emitting the cells that start here.
+  double xl = g->root[0], yl = g->root[1], xh = g->root[2], yh = g->root[3];
+  int depth = 0;
+  const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
+  const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
+  for (int k = 0; k <= internal; ++k) {
+    const bool is_leaf = (k == internal);
+    const int d = is_leaf ? leaf_depth : first_depth + k;
+    while (depth < d) {
+      const double mx = (xl + xh) / 2, my = (yl + yh) / 2;
+      const int quad = (int)((key >> (62 - 2 * depth)) & 3ull);
+      if (quad == 0 || quad == 2) xh = mx; else xl = mx;
+      if (quad >= 2) yh = my; else yl = my;
+      ++depth;
+    }
+    const int id = id0 + k;
+    if ((int64_t)id >= cap) continue;
+    const int end = find_end(L, bmin1, bmin2, n, nb1, nb2, i, d);
+    const int count = end - i + 1;
+    const long long sx = fixpre[end + 1].a - fixpre[i].a;
+    const long long sy = fixpre[end + 1].b - fixpre[i].b;
+    BhNode nd;
+    nd.cx = (float)(r0 + ((double)sx / count) / scale);
+    nd.cy = (float)(r1 + ((double)sy / count) / scale);
+    nd.size = (float)fmax(xh - xl, yh - yl);
+    nd.q = (float)count;
+    nd.skip = (end + 1 < n) ? offset[end + 1] : M;
+    nd.leaf = is_leaf ? (int)vals[i] : -1;
+    nd.count = count;
+    nd.depth = d;
+    nodes[id] = nd;
+  }
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(kBlock) void bh_traverse(
+    int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
+    const float* __restrict__ y, const BhGeometry* __restrict__ g,
+    const BhNode* __restrict__ nodes, float k, float theta, float min_dist,
+    float* __restrict__ fx, float* __restrict__ fy, unsigned long long* __restrict__ stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int p = (int)vals[t];
+  if (g->overflow) {
+    fx[p] = 0.0f;
+    fy[p] = 0.0f;
+    return;
+  }
+  const int M = g->num_nodes;
+  const float px = x[p], py = y[p];
+  const float charge = 1.0f;
+  float ax = 0.0f, ay = 0.0f;
+  unsigned long long inter = 0, opened = 0;
+  int cur = 0;
+  while (cur < M) {
+    const BhNode nd = nodes[cur];
+    if (nd.skip <= cur) break;  // malformed tree: never loop
+    if (nd.leaf == p) {
+      cur = nd.skip;
+      continue;
+    }
+    const float dx = px - nd.cx, dy = py - nd.cy;
+    const float dist_sq = dx * dx + dy * dy;
+    float dist = sqrtf(dist_sq);
+    if (dist < min_dist) dist = min_dist;
+    if (nd.leaf >= 0 || (nd.size / dist < theta)) {
+      const float f = (k * charge * nd.q) / (dist * dist);
+      ax += f * (dx / dist);
+      ay += f * (dy / dist);
+      cur = nd.skip;
+      if (kStats) ++inter;
+    } else {
+      cur = cur + 1;
+      if (kStats) ++opened;
+    }
+  }
+  fx[p] = ax;
+  fy[p] = ay;
+  if (kStats) {
+    atomicAdd(&stats[0], inter);
+    atomicAdd(&stats[1], opened);
+  }
+}
+
+}  // namespace
+
+int bh_alloc(Job& j) {
+  if ((j.p.flags & FX_FLAG_NO_REPULSION)) return FX_OK;
+  BhState& b = j.bh;
+  const int64_t n = j.N;
+  b.nb1 = (n + 255) / 256;
+  b.nb2 = (b.nb1 + 255) / 256;
+  const int64_t scan_elems = scan_tmp_elems(n + 1);
+  int rc;
+  fx_ctx* c = j.ctx;
+  if ((rc = b.part.alloc(c, sizeof(float4) * kReduceBlocks)) ||
+      (rc = b.geom.alloc(c, sizeof(BhGeometry))) || (rc = b.keys.alloc(c, 8 * n)) ||
+      (rc = b.vals.alloc(c, 4 * n)) || (rc = b.lcp.alloc(c, n + 16)) ||
+      (rc = b.bmin1.alloc(c, 4 * b.nb1)) || (rc = b.bmin2.alloc(c, 4 * b.nb2)) ||
+      (rc = b.owned.alloc(c, 4 * n)) || (rc = b.offset.alloc(c, 4 * (n + 1))) ||
+      (rc = b.sxp.alloc(c, 16 * n)) || (rc = b.syp.alloc(c, 16 * (n + 1) + 16 * scan_elems)) ||
+      (rc = b.fx.alloc(c, 4 * n)) || (rc = b.fy.alloc(c, 4 * n)) ||
+      (rc = b.stats.alloc(c, 64)) || (rc = b.sort.ensure(c, n)))
+    return rc;
+  return bh_grow(j, std::max<int64_t>(3 * n, n + 64 * 33) + 1024);
+}
+
+int bh_grow(Job& j, int64_t needed) {
+  BhState& b = j.bh;
+  if (needed <= b.node_cap) needed = b.node_cap * 2;
+  const int64_t cap = needed + needed / 2 + 1024;
+  FX_CHECK(cap < INT32_MAX, FX_E_NOMEM, "Barnes-Hut node count %lld too large", (long long)cap);
+  b.nodes.release();
+  int rc = b.nodes.alloc(j.ctx, sizeof(BhNode) * cap);
+  if (rc) return rc;
+  b.node_cap = cap;
+  return FX_OK;
+}
+
+static int bh_build(Job& j) {
+  BhState& b = j.bh;
+  const int n = j.N;
+  hipStream_t st = j.stream;
+  BhGeometry* g = b.geom.as<BhGeometry>();
+  int rc;
+  if ((rc = phase_begin(j, kPhaseBhKeys))) return rc;
+  hipLaunchKernelGGL(bh_bbox_partial, dim3(kReduceBlocks), dim3(kBlock), 0, st, n,
+                     j.x.as<float>(), j.y.as<float>(), b.part.as<float4>());
+  hipLaunchKernelGGL(bh_geometry, dim3(1), dim3(64), 0, st, b.part.as<float4>(), kReduceBlocks, n, g);
+  FX_HIP(hipGetLastError());
+  if ((rc = fill_i32(st, b.bmin2.as<int32_t>(), b.nb2, 127))) return rc;
+  hipLaunchKernelGGL(bh_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n, j.x.as<float>(),
+                     j.y.as<float>(), (const BhGeometry*)g, b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
+  FX_HIP(hipGetLastError());
+  if ((rc = phase_end(j, kPhaseBhKeys)) || (rc = phase_begin(j, kPhaseBhSort))) return rc;
+  if ((rc = radix_sort_pairs(j.ctx, st, b.sort, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), n, 0, 64)))
+    return rc;
+  if ((rc = phase_end(j, kPhaseBhSort)) || (rc = phase_begin(j, kPhaseBhTree))) return rc;
+  hipLaunchKernelGGL(bh_lcp_count, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
+                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
+                     (const BhGeometry*)g, b.lcp.as<int8_t>(), b.bmin1.as<int>(), b.bmin2.as<int>(),
+                     b.owned.as<int>(), b.sxp.as<I64x2>());
+  FX_HIP(hipGetLastError());
+  const int64_t scan_elems = scan_tmp_elems(n + 1);
+  I64x2* fixpre = b.syp.as<I64x2>();
+  I64x2* tmp = fixpre + (n + 1);
+  // The int32 scan's tile sums share the 16-byte scratch area.
+  if ((rc = scan_exclusive_i32(st, b.owned.as<int32_t>(), b.offset.as<int32_t>(), n,
+                               reinterpret_cast<int32_t*>(tmp), scan_elems)))
+    return rc;
+  if ((rc = scan_exclusive_i64x2(st, b.sxp.p, fixpre, n, tmp, scan_elems))) return rc;
+  hipLaunchKernelGGL(bh_emit, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
+                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), b.lcp.as<int8_t>(),
+                     b.bmin1.as<int>(), b.bmin2.as<int>(), (int)b.nb1, (int)b.nb2,
+                     b.offset.as<int>(), (const I64x2*)fixpre, g, b.nodes.as<BhNode>(), b.node_cap);
+  FX_HIP(hipGetLastError());
+  return phase_end(j, kPhaseBhTree);
+}
+
+int bh_forces(Job& j) {
+  int rc = bh_build(j);
+  if (rc) return rc;
+  BhState& b = j.bh;
+  if ((rc = phase_begin(j, kPhaseBhWalk))) return rc;
+  hipLaunchKernelGGL(bh_traverse<false>, dim3(div_up(j.N, kBlock)), dim3(kBlock), 0, j.stream, j.N,
+                     b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
+                     (const BhGeometry*)b.geom.p, b.nodes.as<BhNode>(), j.p.repel_weight, j.p.theta,
+                     j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(), (unsigned long long*)nullptr);
+  FX_HIP(hipGetLastError());
+  return phase_end(j, kPhaseBhWalk);
+}
+
+}  // namespace fx
+
+extern "C" int fx_job_bh_stats(fx_job* job, int64_t* stats) {
+  fx::clear_error();
+  FX_CHECK(job && stats, FX_E_INVALID, "fx_job_bh_stats: NULL argument");
+  fx::Job& j = job->j;
+  FX_CHECK(!(j.p.flags & FX_FLAG_NO_REPULSION), FX_E_STATE, "job has repulsion disabled");
+  fx::BhState& b = j.bh;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    FX_HIP(hipMemsetAsync(b.stats.p, 0, 64, j.stream));
+    int rc = fx::bh_build(j);
+    if (rc) return rc;
+    hipLaunchKernelGGL(fx::bh_traverse<true>, dim3(fx::div_up(j.N, fx::kBlock)), dim3(fx::kBlock), 0,
+                       j.stream, j.N, b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
+                       (const fx::BhGeometry*)b.geom.p, b.nodes.as<fx::BhNode>(), j.p.repel_weight,
+                       j.p.theta, j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(),
+                       b.stats.as<unsigned long long>());
+    FX_HIP(hipGetLastError());
+    fx::BhGeometry g;
+    unsigned long long st[2];
+    FX_HIP(hipMemcpyAsync(&g, b.geom.p, sizeof(g), hipMemcpyDeviceToHost, j.stream));
+    FX_HIP(hipMemcpyAsync(st, b.stats.p, sizeof(st), hipMemcpyDeviceToHost, j.stream));
+    FX_HIP(hipStreamSynchronize(j.stream));
+    if (g.overflow) {
+      if ((rc = fx::bh_grow(j, g.num_nodes))) return rc;
+      continue;
+    }
+    stats[0] = g.num_nodes;
+    stats[1] = g.depth;
+    stats[2] = (int64_t)st[0];
+    stats[3] = (int64_t)st[1];
+    return FX_OK;
+  }
+  return fx::fail(FX_E_NOMEM, "Barnes-Hut node storage kept overflowing");
+}

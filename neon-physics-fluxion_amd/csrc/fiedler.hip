@@ -1,0 +1,207 @@
+// Power iteration of compute_fiedler_vector
+// (HAPR_fluxion_v4/src/algorithm/partitioner.rs:106-185).
+//
+// The subgraph Laplacian is built on the host as the reference builds it
+// (116-137): local ids in `gates` order, each row's neighbours in adjacency
+// order restricted to the subgraph, degree = number of such entries. Per
+// iteration on the GPU:
+//   lv[i] = deg[i]*v[i] - sum_j v[j]   FP32, row order = the reference's
+//                                       adj_pairs order (159-165)
+//   mean  = sum(lv)/n                  FP64, deterministic two-stage (168-171)
+//   lv   -= (float)mean
+//   norm  = sqrt(sum(lv^2))            FP64 (174-179)
+//   v     = lv / norm  if norm > 1e-10
+// Three launches per iteration; the reductions use a fixed grid and tree, so
+// results are reproducible run to run.
+
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+namespace fx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kGrid = kReduceBlocks;
+
+__device__ __forceinline__ double block_sum_d(double v) {
+  __shared__ double sh[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kBlock / 64; ++k) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+// Every block re-reduces the kGrid partials in the same order, so all see
+// the same total.
+__device__ __forceinline__ double total_of(const double* part) {
+  __shared__ double tot;
+  double v = (threadIdx.x < kGrid) ? part[threadIdx.x] : 0.0;
+  const double r = block_sum_d(v);
+  if (threadIdx.x == 0) tot = r;
+  __syncthreads();
+  return tot;
+}
+
+__global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int* __restrict__ rp,
+                                                      const int* __restrict__ col,
+                                                      const float* __restrict__ deg,
+                                                      const float* __restrict__ v,
+                                                      float* __restrict__ lv, double* part) {
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float a = deg[i] * v[i];
+    const int e = rp[i + 1];
+    for (int k = rp[i]; k < e; ++k) a -= v[col[k]];
+    lv[i] = a;
+    s += (double)a;
+  }
+  const double r = block_sum_d(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void center_kernel(int n, float* __restrict__ lv,
+                                                        const double* part_sum, double* part_sq) {
+  const float mean = (float)(total_of(part_sum) / (double)n);
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float t = lv[i] - mean;
+    lv[i] = t;
+    s += (double)t * (double)t;
+  }
+  const double r = block_sum_d(s);
+  if (threadIdx.x == 0) part_sq[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void sumsq_kernel(int n, const float* __restrict__ v,
+                                                       double* part_sq) {
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += (double)v[i] * (double)v[i];
+  const double r = block_sum_d(s);
+  if (threadIdx.x == 0) part_sq[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void normalise_kernel(int n, const float* __restrict__ lv,
+                                                           const double* part_sq,
+                                                           float* __restrict__ v) {
+  const double nrm = sqrt(total_of(part_sq));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v[i] = (nrm > 1e-10) ? (float)((double)lv[i] / nrm) : lv[i];
+}
+
+// v^T L v and v^T v in FP64.
+__global__ __launch_bounds__(kBlock) void rayleigh_kernel(int n, const int* __restrict__ rp,
+                                                          const int* __restrict__ col,
+                                                          const float* __restrict__ deg,
+                                                          const float* __restrict__ v,
+                                                          double* part_num, double* part_den) {
+  double a = 0.0, b = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double l = (double)deg[i] * v[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) l -= (double)v[col[k]];
+    a += (double)v[i] * l;
+    b += (double)v[i] * v[i];
+  }
+  const double ra = block_sum_d(a);
+  const double rb = block_sum_d(b);
+  if (threadIdx.x == 0) {
+    part_num[blockIdx.x] = ra;
+    part_den[blockIdx.x] = rb;
+  }
+}
+
+__global__ void finish2(const double* pa, const double* pb, double* out) {
+  const double a = total_of(pa);
+  const double b = total_of(pb);
+  if (threadIdx.x == 0) out[0] = b > 0.0 ? a / b : 0.0;
+}
+
+}  // namespace
+}  // namespace fx
+
+extern "C" int fx_fiedler(fx_ctx* ctx, int32_t n, const int32_t* gates, int32_t num_gates,
+                          const int32_t* adj_ptr, const int32_t* adj, const float* v0,
+                          int32_t iterations, float* v_out, double* rayleigh) {
+  fx::clear_error();
+  FX_CHECK(ctx && gates && adj_ptr && adj && v0 && v_out && n >= 0 && iterations >= 0,
+           FX_E_INVALID, "fx_fiedler: bad arguments");
+  if (n == 0) return FX_OK;
+  // partitioner.rs:111-114: n <= 1 returns zeros.
+  if (n == 1) {
+    v_out[0] = 0.0f;
+    if (rayleigh) *rayleigh = 0.0;
+    return FX_OK;
+  }
+  std::vector<int32_t> g2l(num_gates, -1);
+  for (int i = 0; i < n; ++i) {
+    FX_CHECK(gates[i] >= 0 && gates[i] < num_gates, FX_E_INVALID, "gate %d out of range", gates[i]);
+    g2l[gates[i]] = i;
+  }
+  std::vector<int32_t> rp(n + 1, 0), col;
+  std::vector<float> deg(n, 0.0f);
+  for (int li = 0; li < n; ++li) {
+    const int g = gates[li];
+    for (int k = adj_ptr[g]; k < adj_ptr[g + 1]; ++k) {
+      const int nb = adj[k];
+      if (nb < 0 || nb >= num_gates) continue;
+      const int lj = g2l[nb];
+      if (lj < 0) continue;
+      col.push_back(lj);
+      deg[li] += 1.0f;
+    }
+    rp[li + 1] = (int32_t)col.size();
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  FX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  fx::DevBuf drp, dcol, ddeg, dv, dlv, dpart;
+  int rc;
+  if ((rc = drp.alloc(ctx, 4 * (size_t)(n + 1))) || (rc = dcol.alloc(ctx, 4 * (col.size() + 1))) ||
+      (rc = ddeg.alloc(ctx, 4 * (size_t)n)) || (rc = dv.alloc(ctx, 4 * (size_t)n)) ||
+      (rc = dlv.alloc(ctx, 4 * (size_t)n)) || (rc = dpart.alloc(ctx, 8 * 3 * fx::kGrid + 64)))
+    return rc;
+  FX_HIP(hipMemcpyAsync(drp.p, rp.data(), 4 * (size_t)(n + 1), hipMemcpyHostToDevice, st));
+  if (!col.empty())
+    FX_HIP(hipMemcpyAsync(dcol.p, col.data(), 4 * col.size(), hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(ddeg.p, deg.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(dlv.p, v0, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  double* psum = dpart.as<double>();
+  double* psq = psum + fx::kGrid;
+  double* pden = psq + fx::kGrid;
+  double* out = pden + fx::kGrid;
+  const dim3 grid(fx::kGrid), block(fx::kBlock);
+  // Initial normalisation (partitioner.rs:149-155).
+  hipLaunchKernelGGL(fx::sumsq_kernel, grid, block, 0, st, n, dlv.as<float>(), psq);
+  hipLaunchKernelGGL(fx::normalise_kernel, grid, block, 0, st, n, dlv.as<float>(), psq, dv.as<float>());
+  FX_HIP(hipGetLastError());
+  FX_HIP(hipEventRecord(ctx->ev0, st));
+  for (int it = 0; it < iterations; ++it) {
+    hipLaunchKernelGGL(fx::spmv_kernel, grid, block, 0, st, n, drp.as<int>(), dcol.as<int>(),
+                       ddeg.as<float>(), dv.as<float>(), dlv.as<float>(), psum);
+    hipLaunchKernelGGL(fx::center_kernel, grid, block, 0, st, n, dlv.as<float>(), psum, psq);
+    hipLaunchKernelGGL(fx::normalise_kernel, grid, block, 0, st, n, dlv.as<float>(), psq, dv.as<float>());
+  }
+  FX_HIP(hipGetLastError());
+  FX_HIP(hipEventRecord(ctx->ev1, st));
+  if (rayleigh) {
+    hipLaunchKernelGGL(fx::rayleigh_kernel, grid, block, 0, st, n, drp.as<int>(), dcol.as<int>(),
+                       ddeg.as<float>(), dv.as<float>(), psum, pden);
+    hipLaunchKernelGGL(fx::finish2, dim3(1), block, 0, st, psum, pden, out);
+    FX_HIP(hipGetLastError());
+    FX_HIP(hipMemcpyAsync(rayleigh, out, 8, hipMemcpyDeviceToHost, st));
+  }
+  FX_HIP(hipMemcpyAsync(v_out, dv.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+  FX_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  FX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->fiedler_ms = ms;
+  return FX_OK;
+}
+
+extern "C" double fx_fiedler_last_ms(const fx_ctx* ctx) { return ctx ? ctx->fiedler_ms : 0.0; }

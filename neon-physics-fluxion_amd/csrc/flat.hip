@@ -1,0 +1,915 @@
+// Flat global placement job: device-resident netlist and state, net springs,
+// the fused force-combine + integrator, congestion grid, HPWL and overflow.
+//
+// Net springs (placer.rs:242-255 written per net): a per-net segmented
+// reduction over the net's pins gives the centroid c_e and weight w*k; a
+// per-gate segmented reduction over the gate's incident nets (ascending net
+// id) accumulates w*k*(c_e - x_i) — the clique force sum_j w*(x_j - x_i) of
+// the net through its centroid, which is also the star force with weight
+// w*k. Both reductions read CSR arrays in order; no float atomics.
+//
+// Congestion (congestion.rs:11-62): per net, the bounding box cells of
+// RoutingGrid::cell_at (grid.rs:58-62) become 4 corner deltas of a
+// (rows+1)x(cols+1) int32 grid (LDS-privatised for small grids, global
+// atomics otherwise); a 2-D inclusive prefix sum yields the count of nets
+// covering each cell. Integer throughout, so bit-exact.
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "job.h"
+
+namespace fx {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void net_centroid_kernel(int E, const int* __restrict__ net_ptr,
+                                    const int* __restrict__ net_pins,
+                                    const float* __restrict__ x, const float* __restrict__ y,
+                                    float w, float4* __restrict__ netc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int s = net_ptr[e], t = net_ptr[e + 1];
+  float sx = 0.0f, sy = 0.0f;
+  for (int p = s; p < t; ++p) {
+    const int g = net_pins[p];
+    sx += x[g];
+    sy += y[g];
+  }
+  const float deg = (float)(t - s);
+  netc[e] = make_float4(sx / deg, sy / deg, (t - s >= 2) ? w * deg : 0.0f, 0.0f);
+}
+
+// Density field at a gate's bin: E = -grad(phi) by the centred differences
+// of force_electrostatic.py:93-103, evaluated from phi on the fly.
+__device__ __forceinline__ void density_at(const float* __restrict__ phi, int G, double bin_w,
+                                           double bin_h, float wx, float wy, float xi, float yi,
+                                           float& ex, float& ey) {
+  const double qc = fmin(fmax((double)xi / bin_w, 0.0), (double)(G - 1));
+  const double qr = fmin(fmax((double)yi / bin_h, 0.0), (double)(G - 1));
+  const int c = (int)qc, r = (int)qr;
+  const float* row = phi + (size_t)r * G;
+  if (c == 0) ex = -(row[1] - row[0]) * wx;
+  else if (c == G - 1) ex = -(row[c] - row[c - 1]) * wx;
+  else ex = -(row[c + 1] - row[c - 1]) * wx * 0.5f;
+  if (r == 0) ey = -(phi[(size_t)G + c] - phi[c]) * wy;
+  else if (r == G - 1) ey = -(row[c] - phi[(size_t)(r - 1) * G + c]) * wy;
+  else ey = -(phi[(size_t)(r + 1) * G + c] - phi[(size_t)(r - 1) * G + c]) * wy * 0.5f;
+}
+
+struct CombineArgs {
+  int N;
+  const int* gate_ptr;
+  const int* gate_net;
+  const float4* netc;
+  const float* bfx;
+  const float* bfy;
+  const float* phi;
+  int G;
+  double bin_w, bin_h;
+  float wx, wy, dens_w;
+  int comps;
+  int integrate;
+  float dt, damping, die_w, die_h;
+  float* x;
+  float* y;
+  float* vx;
+  float* vy;
+  float* fxo;
+  float* fyo;
+};
+
+// f = (springs + Barnes-Hut) + density, then (optionally) the damped Euler
+// step of placer.rs:274-280 clamped to the die [0, die_w] x [0, die_h].
+__global__ void combine_kernel(CombineArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const float xi = a.x[i], yi = a.y[i];
+  float sx = 0.0f, sy = 0.0f;
+  if (a.comps & 1) {
+    const int q1 = a.gate_ptr[i + 1];
+    for (int q = a.gate_ptr[i]; q < q1; ++q) {
+      const float4 c = a.netc[a.gate_net[q]];
+      sx += c.z * (c.x - xi);
+      sy += c.z * (c.y - yi);
+    }
+  }
+  float bx = 0.0f, by = 0.0f;
+  if (a.comps & 2) {
+    bx = a.bfx[i];
+    by = a.bfy[i];
+  }
+  float dx = 0.0f, dy = 0.0f;
+  if (a.comps & 4) {
+    float ex, ey;
+    density_at(a.phi, a.G, a.bin_w, a.bin_h, a.wx, a.wy, xi, yi, ex, ey);
+    dx = a.dens_w * ex;
+    dy = a.dens_w * ey;
+  }
+  const float gx = (sx + bx) + dx;
+  const float gy = (sy + by) + dy;
+  if (!a.integrate) {
+    a.fxo[i] = gx;
+    a.fyo[i] = gy;
+    return;
+  }
+  const float nvx = (a.vx[i] + gx * a.dt) * a.damping;
+  const float nvy = (a.vy[i] + gy * a.dt) * a.damping;
+  float nx = xi + nvx * a.dt, ny = yi + nvy * a.dt;
+  if (nx < 0.0f) nx = 0.0f;
+  if (nx > 0.0f + a.die_w) nx = 0.0f + a.die_w;
+  if (ny < 0.0f) ny = 0.0f;
+  if (ny > 0.0f + a.die_h) ny = 0.0f + a.die_h;
+  a.vx[i] = nvx;
+  a.vy[i] = nvy;
+  a.x[i] = nx;
+  a.y[i] = ny;
+}
+
+// ---- congestion ----
+__device__ __forceinline__ int cell_index(float q, int n) {
+  // (q as usize).min(n-1) of grid.rs:58-62: NaN and negatives -> 0.
+  if (!(q > 0.0f)) return 0;
+  if (q >= (float)n) return n - 1;
+  return (int)q;
+}
+
+__device__ __forceinline__ bool net_cells(int e, const int* __restrict__ net_ptr,
+                                          const int* __restrict__ net_pins,
+                                          const float* __restrict__ x,
+                                          const float* __restrict__ y, float cw, float ch,
+                                          int cols, int rows, int& c0, int& r0, int& c1,
+                                          int& r1) {
+  const int s = net_ptr[e], t = net_ptr[e + 1];
+  if (t - s < 2) return false;
+  float mnx = FLT_MAX, mxx = -FLT_MAX, mny = FLT_MAX, mxy = -FLT_MAX;
+  for (int p = s; p < t; ++p) {
+    const int g = net_pins[p];
+    mnx = fminf(mnx, x[g]);
+    mxx = fmaxf(mxx, x[g]);
+    mny = fminf(mny, y[g]);
+    mxy = fmaxf(mxy, y[g]);
+  }
+  c0 = cell_index(mnx / cw, cols);
+  r0 = cell_index(mny / ch, rows);
+  c1 = cell_index(mxx / cw, cols);
+  r1 = cell_index(mxy / ch, rows);
+  return c0 <= c1 && r0 <= r1;
+}
+
+__global__ void cong_corners_global(int E, const int* __restrict__ net_ptr,
+                                    const int* __restrict__ net_pins,
+                                    const float* __restrict__ x, const float* __restrict__ y,
+                                    float cw, float ch, int cols, int rows, int* delta) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int c0, r0, c1, r1;
+  if (!net_cells(e, net_ptr, net_pins, x, y, cw, ch, cols, rows, c0, r0, c1, r1)) return;
+  const int W = cols + 1;
+  atomicAdd(&delta[r0 * W + c0], 1);
+  atomicAdd(&delta[r0 * W + c1 + 1], -1);
+  atomicAdd(&delta[(r1 + 1) * W + c0], -1);
+  atomicAdd(&delta[(r1 + 1) * W + c1 + 1], 1);
+}
+
+// LDS-privatised corner histogram: each workgroup counts a strided share of
+// the nets into its own copy, then flushes non-zero entries with atomics.
+constexpr int kLdsCells = 16384;
+__global__ void cong_corners_lds(int E, const int* __restrict__ net_ptr,
+                                 const int* __restrict__ net_pins, const float* __restrict__ x,
+                                 const float* __restrict__ y, float cw, float ch, int cols,
+                                 int rows, int* delta) {
+  __shared__ int h[kLdsCells];
+  const int W = cols + 1, cells = (rows + 1) * W;
+  for (int i = threadIdx.x; i < cells; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+    int c0, r0, c1, r1;
+    if (!net_cells(e, net_ptr, net_pins, x, y, cw, ch, cols, rows, c0, r0, c1, r1)) continue;
+    atomicAdd(&h[r0 * W + c0], 1);
+    atomicAdd(&h[r0 * W + c1 + 1], -1);
+    atomicAdd(&h[(r1 + 1) * W + c0], -1);
+    atomicAdd(&h[(r1 + 1) * W + c1 + 1], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < cells; i += blockDim.x)
+    if (h[i]) atomicAdd(&delta[i], h[i]);
+}
+
+// Row pass of the 2-D inclusive scan: one wave per row.
+__global__ void cong_row_scan(const int* __restrict__ delta, int cols, int rows,
+                              int* __restrict__ out) {
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const int W = cols + 1;
+  const int per = (cols + 63) / 64;
+  const int c0 = lane * per;
+  int local = 0;
+  for (int k = 0; k < per; ++k)
+    if (c0 + k < cols) local += delta[(size_t)row * W + c0 + k];
+  int incl = local;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  int run = incl - local;
+  for (int k = 0; k < per; ++k)
+    if (c0 + k < cols) {
+      run += delta[(size_t)row * W + c0 + k];
+      out[(size_t)row * cols + c0 + k] = run;
+    }
+}
+
+// Column pass: a 1024-thread block covers 64 columns; each of its 16 waves
+// scans a contiguous segment of rows after an exclusive prefix over the
+// segment totals.
+__global__ __launch_bounds__(1024) void cong_col_scan(const int* __restrict__ in, int cols,
+                                                      int rows, int* __restrict__ out) {
+  __shared__ int part[16][64];
+  const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const int per = (rows + 15) / 16;
+  const int r0 = seg * per, r1 = min(rows, r0 + per);
+  int s = 0;
+  if (c < cols)
+    for (int r = r0; r < r1; ++r) s += in[(size_t)r * cols + c];
+  part[seg][lane] = s;
+  __syncthreads();
+  int run = 0;
+  for (int k = 0; k < seg; ++k) run += part[k][lane];
+  if (c < cols)
+    for (int r = r0; r < r1; ++r) {
+      run += in[(size_t)r * cols + c];
+      out[(size_t)r * cols + c] = run;
+    }
+}
+
+// ---- deterministic FP64 reductions (fixed grid, fixed tree order) ----
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sh[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kBlock / 64; ++k) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double block_max(double v) {
+  __shared__ double sh[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double r = -INFINITY;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kBlock / 64; ++k) r = fmax(r, sh[k]);
+  __syncthreads();
+  return r;
+}
+
+// Placement::compute_hpwl (placement.rs:48-79): per-net FP64 term, FP64 sum.
+__global__ void hpwl_partial(int E, const int* __restrict__ net_ptr,
+                             const int* __restrict__ net_pins, const float* __restrict__ x,
+                             const float* __restrict__ y, double* part) {
+  double acc = 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+    const int s = net_ptr[e], t = net_ptr[e + 1];
+    if (t - s < 2) continue;
+    float mnx = FLT_MAX, mxx = -FLT_MAX, mny = FLT_MAX, mxy = -FLT_MAX;
+    for (int p = s; p < t; ++p) {
+      const int g = net_pins[p];
+      mnx = fminf(mnx, x[g]);
+      mxx = fmaxf(mxx, x[g]);
+      mny = fminf(mny, y[g]);
+      mxy = fmaxf(mxy, y[g]);
+    }
+    acc += ((double)mxx - (double)mnx) + ((double)mxy - (double)mny);
+  }
+  const double r = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// grid.rs:70-91 with h = v = count and capacity cap: per cell max(2c-cap,0).
+__global__ void overflow_partial(int64_t cells, const int* __restrict__ counts, double cap,
+                                 double* part_sum, double* part_max) {
+  double s = 0.0, m = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double h = (double)counts[i];
+    const double o = fmax((h + h) - cap, 0.0);
+    s += o;
+    m = fmax(m, o);
+  }
+  const double rs = block_sum(s);
+  const double rm = block_max(m);
+  if (threadIdx.x == 0) {
+    part_sum[blockIdx.x] = rs;
+    part_max[blockIdx.x] = rm;
+  }
+}
+
+__global__ void density_bins_kernel(int N, const float* __restrict__ x,
+                                    const float* __restrict__ y, double bin_w, double bin_h,
+                                    int bins, int* counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double qc = fmin(fmax((double)x[i] / bin_w, 0.0), (double)(bins - 1));
+  const double qr = fmin(fmax((double)y[i] / bin_h, 0.0), (double)(bins - 1));
+  atomicAdd(&counts[(int)qr * bins + (int)qc], 1);
+}
+
+// benchmark_runner.py:264-285: utilisation = area / bin_area per bin.
+__global__ void density_overflow_partial(int64_t cells, const int* __restrict__ counts,
+                                         double bin_area, double target, double* part_sum,
+                                         double* part_max) {
+  double s = 0.0, m = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double u = (double)counts[i] / bin_area;
+    s += fmax(u - target, 0.0);
+    m = fmax(m, u);
+  }
+  const double rs = block_sum(s);
+  const double rm = block_max(m);
+  if (threadIdx.x == 0) {
+    part_sum[blockIdx.x] = rs;
+    part_max[blockIdx.x] = rm;
+  }
+}
+
+__global__ void finish_reduce(int n, const double* part_sum, const double* part_max,
+                              double* out) {
+  double s = 0.0, m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    s += part_sum[i];
+    if (part_max) m = fmax(m, part_max[i]);
+  }
+  const double rs = block_sum(s);
+  const double rm = block_max(m);
+  if (threadIdx.x == 0) {
+    out[0] = rs;
+    out[1] = rm;
+  }
+}
+
+}  // namespace
+
+int fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return FX_OK;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks), dim3(kBlock), 0, st, p, n, v);
+  FX_HIP(hipGetLastError());
+  return FX_OK;
+}
+
+int springs_centroids(Job& j) {
+  if (j.E == 0) return FX_OK;
+  hipLaunchKernelGGL(net_centroid_kernel, dim3(div_up(j.E, kBlock)), dim3(kBlock), 0, j.stream,
+                     j.E, j.net_ptr.as<int>(), j.net_pins.as<int>(), j.x.as<float>(),
+                     j.y.as<float>(), j.p.attract_weight, j.netc.as<float4>());
+  FX_HIP(hipGetLastError());
+  return FX_OK;
+}
+
+int combine(Job& j, int comps, bool integrate) {
+  CombineArgs a;
+  a.N = j.N;
+  a.gate_ptr = j.gate_ptr.as<int>();
+  a.gate_net = j.gate_net.as<int>();
+  a.netc = j.netc.as<float4>();
+  a.bfx = j.bh.fx.as<float>();
+  a.bfy = j.bh.fy.as<float>();
+  a.phi = j.dct.phi.as<float>();
+  a.G = j.dct.G;
+  a.bin_w = j.dct.G ? (double)j.die_w / j.dct.G : 1.0;
+  a.bin_h = j.dct.G ? (double)j.die_h / j.dct.G : 1.0;
+  a.wx = (float)(1.0 / a.bin_w);
+  a.wy = (float)(1.0 / a.bin_h);
+  a.dens_w = j.p.density_weight;
+  a.comps = comps;
+  a.integrate = integrate ? 1 : 0;
+  a.dt = j.p.dt;
+  a.damping = j.p.damping;
+  a.die_w = j.die_w;
+  a.die_h = j.die_h;
+  a.x = j.x.as<float>();
+  a.y = j.y.as<float>();
+  a.vx = j.vx.as<float>();
+  a.vy = j.vy.as<float>();
+  a.fxo = j.fxo.as<float>();
+  a.fyo = j.fyo.as<float>();
+  hipLaunchKernelGGL(combine_kernel, dim3(div_up(j.N, kBlock)), dim3(kBlock), 0, j.stream, a);
+  FX_HIP(hipGetLastError());
+  return FX_OK;
+}
+
+static int job_components(const Job& j) {
+  int c = 0;
+  if (!(j.p.flags & FX_FLAG_NO_SPRINGS)) c |= 1;
+  if (!(j.p.flags & FX_FLAG_NO_REPULSION)) c |= 2;
+  if (!(j.p.flags & FX_FLAG_NO_DENSITY) && j.p.density_grid > 0) c |= 4;
+  return c;
+}
+
+int phase_begin(Job& j, int phase) {
+  if (!(j.timing_mask & (1 << phase))) return FX_OK;
+  if (j.ev_used + 2 > j.ev_pool.size()) {
+    for (int k = 0; k < 64; ++k) {
+      hipEvent_t e;
+      FX_HIP(hipEventCreate(&e));
+      j.ev_pool.push_back(e);
+    }
+  }
+  j.ev_phase.push_back(phase);
+  FX_HIP(hipEventRecord(j.ev_pool[j.ev_used], j.stream));
+  j.ev_used += 1;
+  return FX_OK;
+}
+
+int phase_end(Job& j, int phase) {
+  if (!(j.timing_mask & (1 << phase))) return FX_OK;
+  FX_HIP(hipEventRecord(j.ev_pool[j.ev_used], j.stream));
+  j.ev_used += 1;
+  return FX_OK;
+}
+
+static int enqueue_phases(Job& j, int comps, bool integrate) {
+  int rc;
+  if (comps & 1) {
+    if ((rc = phase_begin(j, kPhaseSprings)) || (rc = springs_centroids(j)) ||
+        (rc = phase_end(j, kPhaseSprings)))
+      return rc;
+  }
+  if ((comps & 2) && (rc = bh_forces(j))) return rc;
+  if (comps & 4) {
+    if ((rc = phase_begin(j, kPhaseDct)) || (rc = dct_field(j)) || (rc = phase_end(j, kPhaseDct)))
+      return rc;
+  }
+  if ((rc = phase_begin(j, kPhaseCombine)) || (rc = combine(j, comps, integrate)) ||
+      (rc = phase_end(j, kPhaseCombine)))
+    return rc;
+  return FX_OK;
+}
+
+static void drop_timing(Job& j) {
+  for (hipEvent_t e : j.ev_pool) (void)hipEventDestroy(e);
+  j.ev_pool.clear();
+  j.ev_phase.clear();
+  j.ev_used = 0;
+}
+
+static void drop_graph(Job& j) {
+  if (j.graph_exec) (void)hipGraphExecDestroy(j.graph_exec);
+  if (j.graph) (void)hipGraphDestroy(j.graph);
+  j.graph_exec = nullptr;
+  j.graph = nullptr;
+}
+
+static int capture_graph(Job& j) {
+  drop_graph(j);
+  FX_HIP(hipStreamBeginCapture(j.stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_phases(j, job_components(j), true);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(j.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FX_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+  j.graph = g;
+  e = hipGraphInstantiate(&j.graph_exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(j);
+    return fail(FX_E_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+  }
+  return FX_OK;
+}
+
+static int enqueue_iterations(Job& j, int iters) {
+  const int comps = job_components(j);
+  if (j.use_graph && !j.timing_mask) {
+    if (!j.graph_exec) {
+      int rc = capture_graph(j);
+      if (rc) return rc;
+    }
+    for (int i = 0; i < iters; ++i) FX_HIP(hipGraphLaunch(j.graph_exec, j.stream));
+    return FX_OK;
+  }
+  for (int i = 0; i < iters; ++i) {
+    int rc = enqueue_phases(j, comps, true);
+    if (rc) return rc;
+  }
+  return FX_OK;
+}
+
+// Reads back the Barnes-Hut overflow flag; on overflow grows the node
+// storage, restores the snapshot and reports that a re-run is needed.
+static int check_regrow(Job& j, bool* rerun) {
+  *rerun = false;
+  if (!(job_components(j) & 2) || j.N < 2) return FX_OK;
+  BhGeometry g;
+  FX_HIP(hipMemcpyAsync(&g, j.bh.geom.p, sizeof(g), hipMemcpyDeviceToHost, j.stream));
+  FX_HIP(hipStreamSynchronize(j.stream));
+  if (!g.overflow) return FX_OK;
+  int rc = bh_grow(j, (int64_t)g.num_nodes);
+  if (rc) return rc;
+  *rerun = true;
+  return FX_OK;
+}
+
+static int snapshot(Job& j, bool restore) {
+  const size_t b = sizeof(float) * (size_t)j.N;
+  char* s = j.snap.as<char>();
+  void* dst[4] = {s, s + b, s + 2 * b, s + 3 * b};
+  void* src[4] = {j.x.p, j.y.p, j.vx.p, j.vy.p};
+  for (int k = 0; k < 4; ++k) {
+    if (restore) FX_HIP(hipMemcpyAsync(src[k], dst[k], b, hipMemcpyDeviceToDevice, j.stream));
+    else FX_HIP(hipMemcpyAsync(dst[k], src[k], b, hipMemcpyDeviceToDevice, j.stream));
+  }
+  return FX_OK;
+}
+
+int run_job(Job& j, int iters, bool wait) {
+  if (iters <= 0) return FX_OK;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    int rc;
+    if ((rc = snapshot(j, false))) return rc;
+    if ((rc = enqueue_iterations(j, iters))) return rc;
+    if (!wait) return FX_OK;
+    bool rerun = false;
+    if ((rc = check_regrow(j, &rerun))) return rc;
+    if (!rerun) return FX_OK;
+    drop_graph(j);
+    if ((rc = snapshot(j, true))) return rc;
+  }
+  return fail(FX_E_NOMEM, "Barnes-Hut node storage kept overflowing");
+}
+
+int reduce_finish(hipStream_t st, const double* psum, const double* pmax, int n, double* out) {
+  hipLaunchKernelGGL(finish_reduce, dim3(1), dim3(kBlock), 0, st, n, psum, pmax, out);
+  FX_HIP(hipGetLastError());
+  return FX_OK;
+}
+
+int congestion_counts(Job& j, int cols, int rows, int32_t** out_counts) {
+  const int64_t W = cols + 1, H = rows + 1;
+  int rc;
+  if ((rc = j.cong.alloc(j.ctx, sizeof(int32_t) * (size_t)(W * H))) ||
+      (rc = j.cong_tmp.alloc(j.ctx, sizeof(int32_t) * (size_t)cols * rows * 2)))
+    return rc;
+  int32_t* delta = j.cong.as<int32_t>();
+  int32_t* rowscan = j.cong_tmp.as<int32_t>();
+  int32_t* counts = rowscan + (size_t)cols * rows;
+  if ((rc = fill_i32(j.stream, delta, W * H, 0))) return rc;
+  const float cw = j.die_w / (float)cols, ch = j.die_h / (float)rows;
+  if (j.E > 0) {
+    if (W * H <= kLdsCells) {
+      const unsigned blocks = std::min<unsigned>(div_up(j.E, kBlock), 512);
+      hipLaunchKernelGGL(cong_corners_lds, dim3(blocks), dim3(kBlock), 0, j.stream, j.E,
+                         j.net_ptr.as<int>(), j.net_pins.as<int>(), j.x.as<float>(),
+                         j.y.as<float>(), cw, ch, cols, rows, delta);
+    } else {
+      hipLaunchKernelGGL(cong_corners_global, dim3(div_up(j.E, kBlock)), dim3(kBlock), 0,
+                         j.stream, j.E, j.net_ptr.as<int>(), j.net_pins.as<int>(),
+                         j.x.as<float>(), j.y.as<float>(), cw, ch, cols, rows, delta);
+    }
+    FX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(cong_row_scan, dim3(div_up(rows, 4)), dim3(256), 0, j.stream, delta, cols,
+                     rows, rowscan);
+  FX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(cong_col_scan, dim3(div_up(cols, 64)), dim3(1024), 0, j.stream, rowscan,
+                     cols, rows, counts);
+  FX_HIP(hipGetLastError());
+  *out_counts = counts;
+  return FX_OK;
+}
+
+}  // namespace fx
+
+using fx::Job;
+
+static int job_upload(Job& j, const fx_netlist* net) {
+  const int N = net->num_gates, E = net->num_nets;
+  const int P = net->net_ptr[E];
+  // Gate -> incident nets (ascending net id), a counting sort of the pins.
+  std::vector<int> gptr(N + 1, 0), gnet(P);
+  for (int p = 0; p < P; ++p) gptr[net->net_pins[p] + 1]++;
+  for (int g = 0; g < N; ++g) gptr[g + 1] += gptr[g];
+  std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+  for (int e = 0; e < E; ++e)
+    for (int p = net->net_ptr[e]; p < net->net_ptr[e + 1]; ++p) gnet[fill[net->net_pins[p]]++] = e;
+  int rc;
+  fx_ctx* c = j.ctx;
+  if ((rc = j.net_ptr.alloc(c, 4 * (size_t)(E + 1))) || (rc = j.net_pins.alloc(c, 4 * (size_t)P + 4)) ||
+      (rc = j.gate_ptr.alloc(c, 4 * (size_t)(N + 1))) || (rc = j.gate_net.alloc(c, 4 * (size_t)P + 4)) ||
+      (rc = j.x.alloc(c, 4 * (size_t)N)) || (rc = j.y.alloc(c, 4 * (size_t)N)) ||
+      (rc = j.vx.alloc(c, 4 * (size_t)N)) || (rc = j.vy.alloc(c, 4 * (size_t)N)) ||
+      (rc = j.netc.alloc(c, 16 * (size_t)(E + 1))) || (rc = j.fxo.alloc(c, 4 * (size_t)N)) ||
+      (rc = j.fyo.alloc(c, 4 * (size_t)N)) || (rc = j.snap.alloc(c, 16 * (size_t)N)) ||
+      (rc = j.red.alloc(c, sizeof(double) * 2 * fx::kReduceBlocks)) ||
+      (rc = j.res.alloc(c, sizeof(double) * 8)))
+    return rc;
+  FX_HIP(hipMemcpyAsync(j.net_ptr.p, net->net_ptr, 4 * (size_t)(E + 1), hipMemcpyHostToDevice, j.stream));
+  if (P) {
+    FX_HIP(hipMemcpyAsync(j.net_pins.p, net->net_pins, 4 * (size_t)P, hipMemcpyHostToDevice, j.stream));
+    FX_HIP(hipMemcpyAsync(j.gate_net.p, gnet.data(), 4 * (size_t)P, hipMemcpyHostToDevice, j.stream));
+  }
+  FX_HIP(hipMemcpyAsync(j.gate_ptr.p, gptr.data(), 4 * (size_t)(N + 1), hipMemcpyHostToDevice, j.stream));
+  FX_HIP(hipMemsetAsync(j.x.p, 0, 4 * (size_t)N, j.stream));
+  FX_HIP(hipMemsetAsync(j.y.p, 0, 4 * (size_t)N, j.stream));
+  FX_HIP(hipMemsetAsync(j.vx.p, 0, 4 * (size_t)N, j.stream));
+  FX_HIP(hipMemsetAsync(j.vy.p, 0, 4 * (size_t)N, j.stream));
+  FX_HIP(hipStreamSynchronize(j.stream));
+  return FX_OK;
+}
+
+extern "C" int fx_job_create(fx_ctx* ctx, const fx_netlist* net, const fx_params* params,
+                             fx_job** out) {
+  fx::clear_error();
+  FX_CHECK(ctx && net && params && out, FX_E_INVALID, "fx_job_create: NULL argument");
+  *out = nullptr;
+  FX_CHECK(net->num_gates >= 1 && net->num_nets >= 0, FX_E_INVALID, "bad netlist sizes");
+  FX_CHECK(net->net_ptr && (net->num_nets == 0 || net->net_pins), FX_E_INVALID, "NULL netlist arrays");
+  FX_CHECK(net->net_ptr[0] == 0, FX_E_INVALID, "net_ptr[0] must be 0");
+  FX_CHECK(net->chip_width > 0.f && net->chip_height > 0.f, FX_E_INVALID, "die must be positive");
+  for (int e = 0; e < net->num_nets; ++e)
+    FX_CHECK(net->net_ptr[e + 1] >= net->net_ptr[e], FX_E_INVALID, "net_ptr not monotone at %d", e);
+  for (int p = 0; p < net->net_ptr[net->num_nets]; ++p)
+    FX_CHECK(net->net_pins[p] >= 0 && net->net_pins[p] < net->num_gates, FX_E_INVALID,
+             "pin %d: gate id %d out of range", p, net->net_pins[p]);
+  const int G = params->density_grid;
+  FX_CHECK(G == 0 || (G >= 16 && G <= 2048 && G % 32 == 0), FX_E_INVALID,
+           "density_grid %d: must be 0 or a multiple of 32 in [32, 2048]", G);
+  FX_CHECK(params->dct_passes == 3 || params->dct_passes == 4, FX_E_INVALID,
+           "dct_passes must be 3 or 4");
+  FX_HIP(hipSetDevice(ctx->device));
+  fx_job* job = new fx_job();
+  Job& j = job->j;
+  j.ctx = ctx;
+  j.N = net->num_gates;
+  j.E = net->num_nets;
+  j.P = net->net_ptr[net->num_nets];
+  j.die_w = net->chip_width;
+  j.die_h = net->chip_height;
+  j.p = *params;
+  const int64_t before = ctx->bytes.load();
+  if (hipStreamCreateWithFlags(&j.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete job;
+    return fx::fail(FX_E_DEVICE, "stream creation failed");
+  }
+  int rc = job_upload(j, net);
+  if (!rc) rc = fx::bh_alloc(j);
+  if (!rc && G > 0) rc = fx::dct_alloc(j);
+  if (rc) {
+    fx_job_destroy(job);
+    return rc;
+  }
+  j.bytes_at_create = ctx->bytes.load() - before;
+  *out = job;
+  return FX_OK;
+}
+
+extern "C" void fx_job_destroy(fx_job* job) {
+  if (!job) return;
+  Job& j = job->j;
+  if (j.stream) (void)hipStreamSynchronize(j.stream);
+  fx::drop_graph(j);
+  fx::drop_timing(j);
+  hipStream_t s = j.stream;
+  delete job;  // DevBufs release their memory
+  if (s) (void)hipStreamDestroy(s);
+}
+
+static int copy_pair(fx_job* job, void* a, void* b, const float* ha, const float* hb, float* oa,
+                     float* ob, int64_t n) {
+  fx::clear_error();
+  FX_CHECK(job, FX_E_INVALID, "NULL job");
+  Job& j = job->j;
+  FX_CHECK(n == j.N, FX_E_INVALID, "expected %d values, got %lld", j.N, (long long)n);
+  const size_t bytes = 4 * (size_t)n;
+  if (ha) {
+    FX_CHECK(hb, FX_E_INVALID, "NULL array");
+    FX_HIP(hipMemcpyAsync(a, ha, bytes, hipMemcpyHostToDevice, j.stream));
+    FX_HIP(hipMemcpyAsync(b, hb, bytes, hipMemcpyHostToDevice, j.stream));
+  } else {
+    FX_CHECK(oa && ob, FX_E_INVALID, "NULL array");
+    FX_HIP(hipMemcpyAsync(oa, a, bytes, hipMemcpyDeviceToHost, j.stream));
+    FX_HIP(hipMemcpyAsync(ob, b, bytes, hipMemcpyDeviceToHost, j.stream));
+  }
+  FX_HIP(hipStreamSynchronize(j.stream));
+  return FX_OK;
+}
+
+extern "C" int fx_job_set_positions(fx_job* job, const float* x, const float* y, int64_t n) {
+  if (!x) return fx::fail(FX_E_INVALID, "NULL x");
+  return copy_pair(job, job ? job->j.x.p : nullptr, job ? job->j.y.p : nullptr, x, y, nullptr, nullptr, n);
+}
+extern "C" int fx_job_get_positions(fx_job* job, float* x, float* y, int64_t n) {
+  return copy_pair(job, job ? job->j.x.p : nullptr, job ? job->j.y.p : nullptr, nullptr, nullptr, x, y, n);
+}
+extern "C" int fx_job_set_velocities(fx_job* job, const float* vx, const float* vy, int64_t n) {
+  if (!vx) return fx::fail(FX_E_INVALID, "NULL vx");
+  return copy_pair(job, job ? job->j.vx.p : nullptr, job ? job->j.vy.p : nullptr, vx, vy, nullptr, nullptr, n);
+}
+extern "C" int fx_job_get_velocities(fx_job* job, float* vx, float* vy, int64_t n) {
+  return copy_pair(job, job ? job->j.vx.p : nullptr, job ? job->j.vy.p : nullptr, nullptr, nullptr, vx, vy, n);
+}
+
+extern "C" int fx_job_run(fx_job* job, int32_t iterations) {
+  fx::clear_error();
+  FX_CHECK(job && iterations >= 0, FX_E_INVALID, "fx_job_run: bad arguments");
+  FX_HIP(hipSetDevice(job->j.ctx->device));
+  int rc = fx::run_job(job->j, iterations, true);
+  if (rc) return rc;
+  FX_HIP(hipStreamSynchronize(job->j.stream));
+  return FX_OK;
+}
+
+extern "C" int fx_job_run_async(fx_job* job, int32_t iterations) {
+  fx::clear_error();
+  FX_CHECK(job && iterations >= 0, FX_E_INVALID, "fx_job_run_async: bad arguments");
+  return fx::run_job(job->j, iterations, false);
+}
+
+extern "C" int fx_job_sync(fx_job* job) {
+  fx::clear_error();
+  FX_CHECK(job, FX_E_INVALID, "NULL job");
+  FX_HIP(hipStreamSynchronize(job->j.stream));
+  bool rerun = false;
+  int rc = fx::check_regrow(job->j, &rerun);
+  if (rc) return rc;
+  if (rerun)
+    return fx::fail(FX_E_STATE,
+                    "Barnes-Hut node storage was grown during an asynchronous run; the state "
+                    "was restored to the run's start: submit the iterations again");
+  return FX_OK;
+}
+
+extern "C" int fx_job_use_graph(fx_job* job, int32_t enable) {
+  fx::clear_error();
+  FX_CHECK(job, FX_E_INVALID, "NULL job");
+  job->j.use_graph = enable != 0;
+  if (!enable) fx::drop_graph(job->j);
+  return FX_OK;
+}
+
+extern "C" int fx_job_forces(fx_job* job, int32_t components, float* fx_out, float* fy_out) {
+  fx::clear_error();
+  FX_CHECK(job && fx_out && fy_out, FX_E_INVALID, "fx_job_forces: NULL argument");
+  Job& j = job->j;
+  int comps = components & 7;
+  if (j.p.density_grid <= 0) comps &= ~4;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    int rc = fx::enqueue_phases(j, comps, false);
+    if (rc) return rc;
+    bool rerun = false;
+    if ((comps & 2) && (rc = fx::check_regrow(j, &rerun))) return rc;
+    if (!rerun) break;
+  }
+  FX_HIP(hipMemcpyAsync(fx_out, j.fxo.p, 4 * (size_t)j.N, hipMemcpyDeviceToHost, j.stream));
+  FX_HIP(hipMemcpyAsync(fy_out, j.fyo.p, 4 * (size_t)j.N, hipMemcpyDeviceToHost, j.stream));
+  FX_HIP(hipStreamSynchronize(j.stream));
+  return FX_OK;
+}
+
+extern "C" int fx_congestion(fx_job* job, int32_t cols, int32_t rows, int32_t* counts) {
+  fx::clear_error();
+  FX_CHECK(job && counts && cols > 0 && rows > 0 && cols <= 8192 && rows <= 8192, FX_E_INVALID,
+           "fx_congestion: bad arguments");
+  Job& j = job->j;
+  int32_t* d = nullptr;
+  int rc = fx::congestion_counts(j, cols, rows, &d);
+  if (rc) return rc;
+  FX_HIP(hipMemcpyAsync(counts, d, sizeof(int32_t) * (size_t)cols * rows, hipMemcpyDeviceToHost, j.stream));
+  FX_HIP(hipStreamSynchronize(j.stream));
+  return FX_OK;
+}
+
+extern "C" int fx_hpwl(fx_job* job, double* hpwl) {
+  fx::clear_error();
+  FX_CHECK(job && hpwl, FX_E_INVALID, "fx_hpwl: NULL argument");
+  Job& j = job->j;
+  double* part = j.red.as<double>();
+  hipLaunchKernelGGL(fx::hpwl_partial, dim3(fx::kReduceBlocks), dim3(fx::kBlock), 0, j.stream, j.E,
+                     j.net_ptr.as<int>(), j.net_pins.as<int>(), j.x.as<float>(), j.y.as<float>(), part);
+  FX_HIP(hipGetLastError());
+  int rc = fx::reduce_finish(j.stream, part, nullptr, fx::kReduceBlocks, j.res.as<double>());
+  if (rc) return rc;
+  double r[2];
+  FX_HIP(hipMemcpyAsync(r, j.res.p, sizeof(r), hipMemcpyDeviceToHost, j.stream));
+  FX_HIP(hipStreamSynchronize(j.stream));
+  *hpwl = r[0];
+  return FX_OK;
+}
+
+extern "C" int fx_overflow(fx_job* job, int32_t cols, int32_t rows, double* total, double* maxo) {
+  fx::clear_error();
+  FX_CHECK(job && total && maxo && cols > 0 && rows > 0 && cols <= 8192 && rows <= 8192,
+           FX_E_INVALID, "fx_overflow: bad arguments");
+  Job& j = job->j;
+  int32_t* d = nullptr;
+  int rc = fx::congestion_counts(j, cols, rows, &d);
+  if (rc) return rc;
+  const float cw = j.die_w / (float)cols, ch = j.die_h / (float)rows;
+  const double cap = (double)((cw + ch) * 0.5f);  // grid.rs:31
+  double* ps = j.red.as<double>();
+  double* pm = ps + fx::kReduceBlocks;
+  hipLaunchKernelGGL(fx::overflow_partial, dim3(fx::kReduceBlocks), dim3(fx::kBlock), 0, j.stream,
+                     (int64_t)cols * rows, d, cap, ps, pm);
+  FX_HIP(hipGetLastError());
+  if ((rc = fx::reduce_finish(j.stream, ps, pm, fx::kReduceBlocks, j.res.as<double>()))) return rc;
+  double r[2];
+  FX_HIP(hipMemcpyAsync(r, j.res.p, sizeof(r), hipMemcpyDeviceToHost, j.stream));
+  FX_HIP(hipStreamSynchronize(j.stream));
+  *total = r[0];
+  *maxo = r[1];
+  return FX_OK;
+}
+
+extern "C" int fx_density_overflow(fx_job* job, int32_t bins, double* max_density,
+                                   double* overflow) {
+  fx::clear_error();
+  FX_CHECK(job && max_density && overflow && bins > 0 && bins <= 4096, FX_E_INVALID,
+           "fx_density_overflow: bad arguments");
+  Job& j = job->j;
+  int rc;
+  if ((rc = j.cong_tmp.alloc(j.ctx, std::max<size_t>(j.cong_tmp.bytes, 4 * (size_t)bins * bins))))
+    return rc;
+  int32_t* cnt = j.cong_tmp.as<int32_t>();
+  if ((rc = fx::fill_i32(j.stream, cnt, (int64_t)bins * bins, 0))) return rc;
+  const double bw = (double)j.die_w / bins, bh = (double)j.die_h / bins;
+  hipLaunchKernelGGL(fx::density_bins_kernel, dim3(fx::div_up(j.N, fx::kBlock)), dim3(fx::kBlock), 0,
+                     j.stream, j.N, j.x.as<float>(), j.y.as<float>(), bw, bh, bins, cnt);
+  FX_HIP(hipGetLastError());
+  double* ps = j.red.as<double>();
+  double* pm = ps + fx::kReduceBlocks;
+  hipLaunchKernelGGL(fx::density_overflow_partial, dim3(fx::kReduceBlocks), dim3(fx::kBlock), 0,
+                     j.stream, (int64_t)bins * bins, cnt, bw * bh, 0.8, ps, pm);
+  FX_HIP(hipGetLastError());
+  if ((rc = fx::reduce_finish(j.stream, ps, pm, fx::kReduceBlocks, j.res.as<double>()))) return rc;
+  double r[2];
+  FX_HIP(hipMemcpyAsync(r, j.res.p, sizeof(r), hipMemcpyDeviceToHost, j.stream));
+  FX_HIP(hipStreamSynchronize(j.stream));
+  *overflow = r[0] / ((double)bins * bins);
+  *max_density = r[1];
+  return FX_OK;
+}
+
+extern "C" int fx_dispatch_congestion(fx_ctx* ctx, const fx_netlist* net, const float* x,
+                                      const float* y, int32_t cols, int32_t rows, int32_t* counts) {
+  fx_params p;
+  fx_params_default(&p);
+  p.flags = FX_FLAG_NO_REPULSION | FX_FLAG_NO_DENSITY;
+  fx_job* job = nullptr;
+  int rc = fx_job_create(ctx, net, &p, &job);
+  if (rc) return rc;
+  rc = fx_job_set_positions(job, x, y, net->num_gates);
+  if (!rc) rc = fx_congestion(job, cols, rows, counts);
+  fx_job_destroy(job);
+  return rc;
+}
+
+extern "C" int fx_job_set_timing(fx_job* job, int32_t phase_mask) {
+  fx::clear_error();
+  FX_CHECK(job, FX_E_INVALID, "NULL job");
+  fx::Job& j = job->j;
+  FX_HIP(hipStreamSynchronize(j.stream));
+  j.timing_mask = phase_mask & ((1 << fx::kNumPhases) - 1);
+  j.ev_phase.clear();
+  j.ev_used = 0;
+  return FX_OK;
+}
+
+extern "C" int fx_job_read_timing(fx_job* job, double* total_ms, int64_t* count) {
+  fx::clear_error();
+  FX_CHECK(job && total_ms && count, FX_E_INVALID, "fx_job_read_timing: NULL argument");
+  fx::Job& j = job->j;
+  FX_HIP(hipStreamSynchronize(j.stream));
+  for (int k = 0; k < fx::kNumPhases; ++k) {
+    total_ms[k] = 0.0;
+    count[k] = 0;
+  }
+  for (size_t i = 0; i < j.ev_phase.size(); ++i) {
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, j.ev_pool[2 * i], j.ev_pool[2 * i + 1]));
+    total_ms[j.ev_phase[i]] += ms;
+    count[j.ev_phase[i]] += 1;
+  }
+  j.ev_phase.clear();
+  j.ev_used = 0;
+  return FX_OK;
+}

@@ -1,0 +1,103 @@
+// Device-resident state of one flat placement job and the phase entry points
+// that enqueue its kernels on the job's stream.
+#pragma once
+
+#include "internal.h"
+
+namespace fx {
+
+// Barnes-Hut tree geometry computed on the device (bh.hip); layout shared
+// with the host only for diagnostics.
+struct BhGeometry {
+  double root[4];     // x_min, y_min, x_max, y_max of the square root cell
+  double scale;       // fixed-point scale 2^scale_bits
+  int32_t depth;      // merge depth D
+  int32_t scale_bits;
+  int32_t num_nodes;  // M of the last build
+  int32_t overflow;   // set when M exceeded the node capacity
+};
+
+// One quadtree node in DFS pre-order (32 bytes).
+struct __align__(16) BhNode {
+  float cx, cy;     // charge-weighted centre
+  float size;       // cell size max(w, h)
+  float q;          // total charge (particle count: unit charges)
+  int32_t skip;     // pre-order index of the next node after this subtree
+  int32_t leaf;     // first particle of a leaf (lowest index), -1 if internal
+  int32_t count;
+  int32_t depth;
+};
+
+struct BhState {
+  DevBuf part, geom, keys, vals, lcp, bmin1, bmin2, owned, offset, sxp, syp, nodes, fx, fy,
+      stats;
+  SortWorkspace sort;
+  int64_t node_cap = 0;
+  int64_t nb1 = 0, nb2 = 0;
+};
+
+struct DctState {
+  int G = 0;
+  DevBuf counts;                 // int32 G*G point-in-bin counts
+  DevBuf c_hi, c_lo, ct_hi, ct_lo;  // bf16 splits of C and C^T
+  DevBuf t0, t1, phi;            // f32 G*G work arrays
+  DevBuf inv_eig;                // f32 G*G: -1/lambda (0 at DC)
+};
+
+struct Job {
+  fx_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  int32_t N = 0, E = 0, P = 0;
+  float die_w = 0.f, die_h = 0.f;
+  fx_params p{};
+  DevBuf net_ptr, net_pins, gate_ptr, gate_net;
+  DevBuf x, y, vx, vy;
+  DevBuf netc;      // float4 per net: centroid x, centroid y, w*k, unused
+  DevBuf fxo, fyo;  // force output for diagnostics
+  DevBuf snap;      // x,y,vx,vy snapshot for re-runs after a capacity regrow
+  DevBuf red;       // reduction partials (double)
+  DevBuf cong;      // congestion delta grid / counts
+  DevBuf cong_tmp;
+  DevBuf res;       // small result area (doubles)
+  BhState bh;
+  DctState dct;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool use_graph = false;
+  int64_t bytes_at_create = 0;
+  // Per-phase HIP-event timing (eager launches only): event pairs recorded
+  // on the job stream around the phases selected by timing_mask.
+  int timing_mask = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_phase;  // phase of each recorded pair
+  size_t ev_used = 0;
+};
+
+enum Phase {
+  kPhaseSprings = 0,   // net centroids
+  kPhaseBhKeys = 1,    // bbox, root cell, quadrant keys
+  kPhaseBhSort = 2,    // radix sort of the keys
+  kPhaseBhTree = 3,    // lcp, scans, node emission
+  kPhaseBhWalk = 4,    // traversal
+  kPhaseDct = 5,       // deposit + 4 GEMMs
+  kPhaseCombine = 6,   // springs gather + density gather + integrator
+  kNumPhases = 7
+};
+// Record the start / end of a phase when it is being timed.
+int phase_begin(Job& j, int phase);
+int phase_end(Job& j, int phase);
+
+// Phases (each enqueues on job.stream; no host synchronisation).
+int springs_centroids(Job& j);
+int bh_forces(Job& j);             // writes j.bh.fx/fy (per original index)
+int dct_field(Job& j);             // writes j.dct.phi
+int combine(Job& j, int components, bool integrate);  // forces -> (fxo,fyo) or integrate
+int bh_alloc(Job& j);
+int dct_alloc(Job& j);
+int bh_grow(Job& j, int64_t needed);
+
+}  // namespace fx
+
+struct fx_job {
+  fx::Job j;
+};

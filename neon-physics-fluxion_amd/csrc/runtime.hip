@@ -1,0 +1,115 @@
+// Context, error reporting and device-buffer bookkeeping of libfluxion_hip.
+// GpuContext::new (HAPR_fluxion_v4/src/gpu/context.rs:17-70) becomes
+// fx_ctx_create: it binds one HIP device and one stream of default priority;
+// a missing device is an error returned to the caller (no CPU fallback).
+
+#include <cstdarg>
+#include <cstring>
+
+#include "internal.h"
+
+namespace fx {
+
+static thread_local char g_err[1024] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void clear_error() { g_err[0] = 0; }
+
+int DevBuf::alloc(fx_ctx* c, size_t n) {
+  if (p && bytes >= n) return FX_OK;
+  release();
+  ctx = c;
+  if (n == 0) n = 256;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return fail(FX_E_NOMEM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+  }
+  bytes = n;
+  if (ctx) ctx->bytes += (int64_t)n;
+  return FX_OK;
+}
+
+void DevBuf::release() {
+  if (p) {
+    (void)hipFree(p);
+    if (ctx) ctx->bytes -= (int64_t)bytes;
+  }
+  p = nullptr;
+  bytes = 0;
+}
+
+}  // namespace fx
+
+extern "C" const char* fx_last_error(void) { return fx::g_err; }
+
+extern "C" void fx_params_default(fx_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->force_iterations = 50;  // placer.rs:24-28
+  p->attract_weight = 1.0f;
+  p->repel_weight = 0.5f;
+  p->damping = 0.9f;
+  p->dt = 0.1f;
+  p->theta = 0.5f;           // BarnesHutTree.h:204
+  p->min_dist = 0.1f;        // placer.rs:263
+  p->density_weight = 1.0f;  // force_electrostatic.py:36
+  p->target_density = 0.8f;  // force_electrostatic.py:37
+  p->density_grid = 0;
+  p->dct_passes = 3;
+  p->flags = 0;
+}
+
+extern "C" int fx_ctx_create(int device, fx_ctx** out) {
+  fx::clear_error();
+  if (!out) return fx::fail(FX_E_INVALID, "fx_ctx_create: out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return fx::fail(FX_E_DEVICE, "no HIP device available (%s)",
+                    e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+  }
+  if (device < 0 || device >= count)
+    return fx::fail(FX_E_INVALID, "device %d out of range [0,%d)", device, count);
+  FX_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  FX_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fx::fail(FX_E_DEVICE, "device %d is %s; this library is built for gfx950 only",
+                    device, prop.gcnArchName);
+  fx_ctx* c = new fx_ctx();
+  c->device = device;
+  std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+  c->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fx::fail(FX_E_DEVICE, "stream/event creation failed");
+  }
+  *out = c;
+  return FX_OK;
+}
+
+extern "C" void fx_ctx_destroy(fx_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" const char* fx_ctx_device_name(const fx_ctx* ctx) { return ctx ? ctx->name : ""; }
+
+extern "C" int64_t fx_ctx_bytes_in_use(const fx_ctx* ctx) { return ctx ? ctx->bytes.load() : 0; }

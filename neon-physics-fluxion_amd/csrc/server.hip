@@ -1,0 +1,273 @@
+// Placement serving with continuous batching.
+//
+// A worker thread owns the device work. Requests wait in a FIFO; the worker
+// admits the head request as soon as a slot is free and its estimated HBM
+// footprint fits the budget next to the requests already running, creates a
+// device-resident job for it (netlist, positions, tree and grids stay in
+// HBM), and from then on feeds every running job chunks of iterations on
+// the job's own stream (graph-captured iterations), so the work of several
+// requests overlaps on the GPU. A finished job is read back (positions,
+// FP64 HPWL), destroyed, and its slot and memory are immediately reused by
+// the next request. Jobs share no buffers and use no float atomics, so a
+// request's result is bit-identical whether it ran alone or in a mix.
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+#include "job.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t id = 0;
+  fx_params params{};
+  int32_t N = 0, E = 0;
+  float die_w = 0, die_h = 0;
+  std::vector<int32_t> net_ptr, net_pins;
+  std::vector<float> x, y;
+  int32_t iterations = 0, remaining = 0, last_chunk = 0;
+  int32_t status = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  double hpwl = 0.0;
+  double t_submit = 0, t_start = 0, t_finish = 0;
+  int64_t footprint = 0;
+  fx_job* job = nullptr;
+  bool in_flight = false;
+  std::string error;
+};
+
+// Device bytes a job of this shape allocates (mirrors job_upload, bh_alloc,
+// dct_alloc and the sort workspace).
+int64_t estimate_bytes(int64_t N, int64_t E, int64_t P, int64_t G, bool bh) {
+  int64_t b = 4 * (E + 1) + 8 * P + 4 * (N + 1) + 16 * N + 16 * (E + 1) + 8 * N + 16 * N + 4096;
+  if (bh) {
+    const int64_t cap = std::max<int64_t>(3 * N, N + 64 * 33) + 1024;
+    const int64_t c = cap + cap / 2 + 1024;
+    b += 8 * N + 4 * N + N + 4 * N + 4 * (N + 1) + 16 * N + 32 * (N + 1) + 8 * N + 32 * c +
+         12 * N + 4 * 256 * ((N + 2047) / 2048) + 8192;
+  }
+  if (G > 0) b += G * G * (4 + 8 + 12 + 4);
+  return b + (b >> 4);
+}
+
+}  // namespace
+
+struct fx_server {
+  fx_ctx* ctx = nullptr;
+  fx_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv, done_cv;
+  std::deque<int64_t> queue;
+  std::map<int64_t, std::unique_ptr<Request>> reqs;
+  std::vector<Request*> active;
+  int64_t next_id = 1;
+  int64_t bytes_active = 0;
+  bool stop = false;
+  std::thread worker;
+  Clock::time_point t0 = Clock::now();
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+  void run();
+  bool admit_locked();
+  void finish(Request* r, int rc);
+};
+
+static const int kChunk = 25;
+
+bool fx_server::admit_locked() {
+  bool any = false;
+  while (!queue.empty() && (int)active.size() < cfg.max_active) {
+    Request* r = reqs[queue.front()].get();
+    if (!active.empty() && bytes_active + r->footprint > cfg.hbm_budget) break;
+    queue.pop_front();
+    r->status = 1;
+    r->t_start = now();
+    active.push_back(r);
+    bytes_active += r->footprint;
+    any = true;
+  }
+  return any;
+}
+
+void fx_server::finish(Request* r, int rc) {
+  if (rc == FX_OK) {
+    r->x.assign(r->N, 0.f);
+    r->y.assign(r->N, 0.f);
+    rc = fx_job_get_positions(r->job, r->x.data(), r->y.data(), r->N);
+    if (rc == FX_OK) rc = fx_hpwl(r->job, &r->hpwl);
+  }
+  if (rc != FX_OK) r->error = fx_last_error();
+  fx_job_destroy(r->job);
+  r->job = nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  r->status = rc == FX_OK ? 2 : 3;
+  r->t_finish = now();
+  r->net_ptr.clear();
+  r->net_ptr.shrink_to_fit();
+  r->net_pins.clear();
+  r->net_pins.shrink_to_fit();
+  bytes_active -= r->footprint;
+  active.erase(std::find(active.begin(), active.end(), r));
+  done_cv.notify_all();
+}
+
+void fx_server::run() {
+  (void)hipSetDevice(ctx->device);
+  for (;;) {
+    std::vector<Request*> snapshot;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+      if (stop && queue.empty() && active.empty()) return;
+      admit_locked();
+      snapshot = active;
+    }
+    bool progressed = false;
+    for (Request* r : snapshot) {
+      if (!r->job) {
+        fx_netlist net{r->N, r->E, r->net_ptr.data(), r->net_pins.data(), r->die_w, r->die_h};
+        int rc = fx_job_create(ctx, &net, &r->params, &r->job);
+        if (rc == FX_OK) rc = fx_job_set_positions(r->job, r->x.data(), r->y.data(), r->N);
+        if (rc == FX_OK && cfg.use_graphs) rc = fx_job_use_graph(r->job, 1);
+        if (rc != FX_OK) {
+          finish(r, rc);
+          progressed = true;
+          continue;
+        }
+      }
+      if (!r->in_flight) {
+        const int it = std::min(kChunk, r->remaining);
+        int rc = fx_job_run_async(r->job, it);
+        if (rc != FX_OK) {
+          finish(r, rc);
+          progressed = true;
+          continue;
+        }
+        r->in_flight = true;
+        r->last_chunk = it;
+        r->remaining -= it;
+        progressed = true;
+      }
+    }
+    for (Request* r : snapshot) {
+      if (!r->job || !r->in_flight) continue;
+      hipError_t q = hipStreamQuery(r->job->j.stream);
+      if (q == hipErrorNotReady) continue;
+      r->in_flight = false;
+      progressed = true;
+      int rc = fx_job_sync(r->job);
+      if (rc == FX_E_STATE) {  // tree storage grown: the chunk was rolled back
+        r->remaining += r->last_chunk;
+        continue;
+      }
+      if (rc != FX_OK || r->remaining == 0) finish(r, rc);
+    }
+    if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+extern "C" int fx_server_create(fx_ctx* ctx, const fx_server_config* cfg, fx_server** out) {
+  fx::clear_error();
+  FX_CHECK(ctx && cfg && out, FX_E_INVALID, "fx_server_create: NULL argument");
+  FX_CHECK(cfg->max_active >= 1 && cfg->hbm_budget > 0, FX_E_INVALID, "bad server config");
+  fx_server* s = new fx_server();
+  s->ctx = ctx;
+  s->cfg = *cfg;
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return FX_OK;
+}
+
+extern "C" int fx_server_submit(fx_server* s, const fx_netlist* net, const fx_params* params,
+                                const float* x0, const float* y0, int32_t iterations,
+                                int64_t* id) {
+  fx::clear_error();
+  FX_CHECK(s && net && params && x0 && y0 && id && iterations >= 0, FX_E_INVALID,
+           "fx_server_submit: bad arguments");
+  FX_CHECK(net->num_gates >= 1 && net->num_nets >= 0 && net->net_ptr, FX_E_INVALID, "bad netlist");
+  auto r = std::make_unique<Request>();
+  r->params = *params;
+  r->N = net->num_gates;
+  r->E = net->num_nets;
+  r->die_w = net->chip_width;
+  r->die_h = net->chip_height;
+  const int32_t P = net->net_ptr[net->num_nets];
+  r->net_ptr.assign(net->net_ptr, net->net_ptr + net->num_nets + 1);
+  r->net_pins.assign(net->net_pins, net->net_pins + P);
+  r->x.assign(x0, x0 + r->N);
+  r->y.assign(y0, y0 + r->N);
+  r->iterations = r->remaining = iterations;
+  r->footprint = estimate_bytes(r->N, r->E, P, params->density_grid,
+                                !(params->flags & FX_FLAG_NO_REPULSION));
+  FX_CHECK(r->footprint <= s->cfg.hbm_budget, FX_E_NOMEM,
+           "request needs ~%lld bytes, more than the server's HBM budget",
+           (long long)r->footprint);
+  std::lock_guard<std::mutex> lk(s->mu);
+  r->id = s->next_id++;
+  r->t_submit = s->now();
+  *id = r->id;
+  s->queue.push_back(r->id);
+  s->reqs[r->id] = std::move(r);
+  s->cv.notify_all();
+  return FX_OK;
+}
+
+extern "C" int fx_server_poll(fx_server* s, int64_t id, int32_t* status, float* x, float* y,
+                              double* hpwl, double* t_submit, double* t_start, double* t_finish) {
+  fx::clear_error();
+  FX_CHECK(s && status, FX_E_INVALID, "fx_server_poll: NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  FX_CHECK(it != s->reqs.end(), FX_E_INVALID, "unknown request id %lld", (long long)id);
+  Request* r = it->second.get();
+  *status = r->status;
+  if (t_submit) *t_submit = r->t_submit;
+  if (t_start) *t_start = r->t_start;
+  if (t_finish) *t_finish = r->t_finish;
+  if (r->status == 3) return fx::fail(FX_E_DEVICE, "request failed: %s", r->error.c_str());
+  if (r->status == 2) {
+    if (x && y && !r->x.empty()) {
+      std::copy(r->x.begin(), r->x.end(), x);
+      std::copy(r->y.begin(), r->y.end(), y);
+    }
+    if (hpwl) *hpwl = r->hpwl;
+  }
+  return FX_OK;
+}
+
+extern "C" int fx_server_release(fx_server* s, int64_t id) {
+  fx::clear_error();
+  FX_CHECK(s, FX_E_INVALID, "NULL server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  FX_CHECK(it != s->reqs.end(), FX_E_INVALID, "unknown request id %lld", (long long)id);
+  FX_CHECK(it->second->status >= 2, FX_E_BUSY, "request %lld not finished", (long long)id);
+  s->reqs.erase(it);
+  return FX_OK;
+}
+
+extern "C" int fx_server_drain(fx_server* s) {
+  fx::clear_error();
+  FX_CHECK(s, FX_E_INVALID, "NULL server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->done_cv.wait(lk, [&] { return s->queue.empty() && s->active.empty(); });
+  return FX_OK;
+}
+
+extern "C" double fx_server_clock(const fx_server* s) { return s ? s->now() : 0.0; }
+
+extern "C" void fx_server_destroy(fx_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->queue.clear();
+    s->cv.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  delete s;
+}

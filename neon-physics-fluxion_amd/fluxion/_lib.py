@@ -1,0 +1,190 @@
+"""ctypes binding of libfluxion_hip.so (the C ABI declared in include/fluxion_hip.h).
+
+The shared library is built in-tree next to this file. There is no fallback:
+if it is missing or fails to load, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfluxion_hip.so")
+
+FX_OK = 0
+FX_E_INVALID = -1
+FX_E_DEVICE = -2
+FX_E_NOMEM = -3
+FX_E_STATE = -4
+FX_E_BUSY = -5
+
+FX_FLAG_NO_SPRINGS = 1
+FX_FLAG_NO_REPULSION = 2
+FX_FLAG_NO_DENSITY = 4
+
+FX_LEAF_EXACT = 0
+FX_LEAF_FAST = 1
+
+
+class FluxionError(RuntimeError):
+    """A libfluxion_hip call returned a non-zero status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class FxParams(C.Structure):
+    _fields_ = [
+        ("force_iterations", C.c_int32),
+        ("attract_weight", C.c_float),
+        ("repel_weight", C.c_float),
+        ("damping", C.c_float),
+        ("dt", C.c_float),
+        ("theta", C.c_float),
+        ("min_dist", C.c_float),
+        ("density_weight", C.c_float),
+        ("target_density", C.c_float),
+        ("density_grid", C.c_int32),
+        ("dct_passes", C.c_int32),
+        ("flags", C.c_int32),
+    ]
+
+
+class FxNetlist(C.Structure):
+    _fields_ = [
+        ("num_gates", C.c_int32),
+        ("num_nets", C.c_int32),
+        ("net_ptr", C.POINTER(C.c_int32)),
+        ("net_pins", C.POINTER(C.c_int32)),
+        ("chip_width", C.c_float),
+        ("chip_height", C.c_float),
+    ]
+
+
+class FxLeafBatch(C.Structure):
+    _fields_ = [
+        ("num_leaves", C.c_int32),
+        ("leaf_ptr", C.POINTER(C.c_int32)),
+        ("leaf_gates", C.POINTER(C.c_int32)),
+        ("region", C.POINTER(C.c_float)),
+        ("num_gates", C.c_int32),
+        ("adj_ptr", C.POINTER(C.c_int32)),
+        ("adj", C.POINTER(C.c_int32)),
+    ]
+
+
+class FxServerConfig(C.Structure):
+    _fields_ = [
+        ("max_active", C.c_int32),
+        ("hbm_budget", C.c_int64),
+        ("use_graphs", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_F = C.c_float
+_D = C.c_double
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "fx_params_default": (None, [C.POINTER(FxParams)]),
+    "fx_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "fx_ctx_destroy": (None, [_P]),
+    "fx_last_error": (C.c_char_p, []),
+    "fx_ctx_device_name": (C.c_char_p, [_P]),
+    "fx_ctx_bytes_in_use": (_I64, [_P]),
+    "fx_place_leaves": (C.c_int, [_P, C.POINTER(FxLeafBatch), C.POINTER(FxParams), _I32, _P, _P]),
+    "fx_place_leaves_state": (C.c_int, [_P, C.POINTER(FxLeafBatch), C.POINTER(FxParams), _I32, _I32,
+                                        _I32, _P, _P, _P, _P, _P, _P]),
+    "fx_leaves_last_kernel_ms": (_D, [_P]),
+    "fx_job_create": (C.c_int, [_P, C.POINTER(FxNetlist), C.POINTER(FxParams), C.POINTER(_P)]),
+    "fx_job_destroy": (None, [_P]),
+    "fx_job_set_positions": (C.c_int, [_P, _P, _P, _I64]),
+    "fx_job_get_positions": (C.c_int, [_P, _P, _P, _I64]),
+    "fx_job_set_velocities": (C.c_int, [_P, _P, _P, _I64]),
+    "fx_job_get_velocities": (C.c_int, [_P, _P, _P, _I64]),
+    "fx_job_run": (C.c_int, [_P, _I32]),
+    "fx_job_forces": (C.c_int, [_P, _I32, _P, _P]),
+    "fx_job_density_fields": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fx_job_bh_stats": (C.c_int, [_P, _P]),
+    "fx_job_run_async": (C.c_int, [_P, _I32]),
+    "fx_job_sync": (C.c_int, [_P]),
+    "fx_job_use_graph": (C.c_int, [_P, _I32]),
+    "fx_job_set_timing": (C.c_int, [_P, _I32]),
+    "fx_job_read_timing": (C.c_int, [_P, _P, _P]),
+    "fx_congestion": (C.c_int, [_P, _I32, _I32, _P]),
+    "fx_hpwl": (C.c_int, [_P, C.POINTER(_D)]),
+    "fx_overflow": (C.c_int, [_P, _I32, _I32, C.POINTER(_D), C.POINTER(_D)]),
+    "fx_density_overflow": (C.c_int, [_P, _I32, C.POINTER(_D), C.POINTER(_D)]),
+    "fx_dispatch_congestion": (C.c_int, [_P, C.POINTER(FxNetlist), _P, _P, _I32, _I32, _P]),
+    "fx_fiedler": (C.c_int, [_P, _I32, _P, _I32, _P, _P, _P, _I32, _P, C.POINTER(_D)]),
+    "fx_fiedler_last_ms": (_D, [_P]),
+    "fx_sort_u64": (C.c_int, [_P, _I64, _P, _P]),
+    "fx_sort_f32": (C.c_int, [_P, _I64, _P, _P]),
+    "fx_split_by_fiedler": (C.c_int, [_P, _I32, _P, _P, _F, _P, C.POINTER(_I32)]),
+    "fx_server_create": (C.c_int, [_P, C.POINTER(FxServerConfig), C.POINTER(_P)]),
+    "fx_server_submit": (C.c_int, [_P, C.POINTER(FxNetlist), C.POINTER(FxParams), _P, _P, _I32,
+                                   C.POINTER(_I64)]),
+    "fx_server_poll": (C.c_int, [_P, _I64, C.POINTER(_I32), _P, _P, C.POINTER(_D), C.POINTER(_D),
+                                 C.POINTER(_D), C.POINTER(_D)]),
+    "fx_server_release": (C.c_int, [_P, _I64]),
+    "fx_server_drain": (C.c_int, [_P]),
+    "fx_server_clock": (_D, [_P]),
+    "fx_server_destroy": (None, [_P]),
+    "fx_build_adjacency": (_I64, [_I32, _I32, _P, _P, _P, _P, _I64]),
+    "fx_netgen_flat": (_I64, [_I32, _I32, C.c_uint64, _F, _F, _P, _P, _I64, _P, _P]),
+    "fx_netgen_leaves": (_I64, [_I32, C.c_uint64, _I32, _I32, _F, _P, _P, _P, _P, _P]),
+}
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+SYMBOLS = tuple(_SIGS)
+
+
+def check(rc: int) -> None:
+    if rc != FX_OK:
+        msg = lib.fx_last_error()
+        raise FluxionError(rc, msg.decode() if msg else "unknown error")
+
+
+def ptr(a: np.ndarray | None):
+    """Raw pointer of a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def i32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def f32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def default_params(**kw) -> FxParams:
+    p = FxParams()
+    lib.fx_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"fx_params has no field {k}")
+        setattr(p, k, v)
+    return p

@@ -1,0 +1,469 @@
+"""Host-side mirror of the reference's placement operator API on the MI355X library.
+
+Reference interface (HAPR_fluxion_v4):
+  * PlacerConfig                       src/algorithm/placer.rs:13-31
+  * hapr_placement leaf loop            src/algorithm/placer.rs:66-69 -> place_leaves
+  * place_gates_in_leaf                 src/algorithm/placer.rs:210-287
+  * GpuContext::new() -> Option         src/gpu/context.rs:17-19       -> GpuContext.new()
+  * dispatch_forces                     src/gpu/dispatch.rs:18-27      -> dispatch_forces / FlatJob
+  * dispatch_congestion                 src/gpu/dispatch.rs:109-114    -> dispatch_congestion
+  * Circuit::build_adjacency            src/data/circuit.rs:154-171    -> Circuit.build_adjacency
+  * Placement::compute_hpwl             src/data/placement.rs:48-79    -> FlatJob.hpwl
+  * RoutingGrid overflow                src/data/grid.rs:70-91         -> FlatJob.overflow
+  * compute_fiedler_vector              src/algorithm/partitioner.rs:106-185 -> compute_fiedler_vector
+  * split_by_fiedler                    src/algorithm/partitioner.rs:189-213 -> split_by_fiedler
+
+Every call runs on the GPU through libfluxion_hip.so; errors raise FluxionError
+(the reference returns None / Result::Err or panics).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import (FX_FLAG_NO_DENSITY, FX_FLAG_NO_REPULSION, FX_FLAG_NO_SPRINGS, FX_LEAF_EXACT,
+                   FX_LEAF_FAST, FluxionError, FxLeafBatch, FxNetlist, FxParams, FxServerConfig,
+                   check, default_params, f32p, i32p, lib, ptr)
+
+__all__ = [
+    "PlacerConfig", "GpuContext", "Circuit", "Placement", "FlatJob", "PlacementServer",
+    "place_leaves", "dispatch_forces", "dispatch_congestion", "compute_fiedler_vector",
+    "split_by_fiedler", "sort_f32", "sort_u64", "FluxionError",
+]
+
+
+@dataclass
+class PlacerConfig:
+    """PlacerConfig (placer.rs:13-31) plus the flat-loop knobs of fx_params."""
+    force_iterations: int = 50
+    attract_weight: float = 1.0
+    repel_weight: float = 0.5
+    damping: float = 0.9
+    dt: float = 0.1
+    theta: float = 0.5
+    min_dist: float = 0.1
+    density_weight: float = 1.0
+    target_density: float = 0.8
+    density_grid: int = 0
+    dct_passes: int = 3
+    springs: bool = True
+    repulsion: bool = True
+    density: bool = True
+
+    def to_params(self) -> FxParams:
+        flags = 0
+        if not self.springs:
+            flags |= FX_FLAG_NO_SPRINGS
+        if not self.repulsion:
+            flags |= FX_FLAG_NO_REPULSION
+        if not self.density:
+            flags |= FX_FLAG_NO_DENSITY
+        return default_params(
+            force_iterations=int(self.force_iterations), attract_weight=self.attract_weight,
+            repel_weight=self.repel_weight, damping=self.damping, dt=self.dt, theta=self.theta,
+            min_dist=self.min_dist, density_weight=self.density_weight,
+            target_density=self.target_density, density_grid=int(self.density_grid),
+            dct_passes=int(self.dct_passes), flags=flags)
+
+
+class GpuContext:
+    """GpuContext (context.rs:17-70): one device, one stream. No CPU fallback."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def new(cls, device: int = 0) -> "GpuContext":
+        h = C.c_void_p()
+        check(lib.fx_ctx_create(device, C.byref(h)))
+        return cls(h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def adapter_name(self) -> str:
+        return lib.fx_ctx_device_name(self._h).decode()
+
+    def bytes_in_use(self) -> int:
+        return int(lib.fx_ctx_bytes_in_use(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            lib.fx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+@dataclass
+class Circuit:
+    """The netlist part of Circuit (circuit.rs:43-78) as CSR arrays."""
+    num_gates: int
+    net_ptr: np.ndarray
+    net_pins: np.ndarray
+    chip_width: float = 1000.0
+    chip_height: float = 1000.0
+    _adj: tuple | None = field(default=None, repr=False)
+
+    def __post_init__(self):
+        self.net_ptr = np.ascontiguousarray(self.net_ptr, dtype=np.int32)
+        self.net_pins = np.ascontiguousarray(self.net_pins, dtype=np.int32)
+
+    @property
+    def num_nets(self) -> int:
+        return int(self.net_ptr.shape[0] - 1)
+
+    @classmethod
+    def from_nets(cls, num_gates: int, nets, chip_width=1000.0, chip_height=1000.0) -> "Circuit":
+        ptr_ = np.zeros(len(nets) + 1, np.int32)
+        ptr_[1:] = np.cumsum([len(n) for n in nets])
+        pins = np.array([g for n in nets for g in n], np.int32)
+        return cls(num_gates, ptr_, pins, chip_width, chip_height)
+
+    def netlist(self) -> FxNetlist:
+        n = FxNetlist()
+        n.num_gates = self.num_gates
+        n.num_nets = self.num_nets
+        n.net_ptr = i32p(self.net_ptr)
+        n.net_pins = i32p(self.net_pins)
+        n.chip_width = self.chip_width
+        n.chip_height = self.chip_height
+        return n
+
+    def build_adjacency(self) -> tuple[np.ndarray, np.ndarray]:
+        """Circuit::build_adjacency (circuit.rs:154-171) as CSR (adj_ptr, adj)."""
+        if self._adj is None:
+            count = lib.fx_build_adjacency(self.num_gates, self.num_nets, ptr(self.net_ptr),
+                                           ptr(self.net_pins), None, None, 0)
+            if count < 0:
+                raise FluxionError(int(count), "fx_build_adjacency failed")
+            aptr = np.zeros(self.num_gates + 1, np.int32)
+            adj = np.zeros(max(1, count), np.int32)
+            rc = lib.fx_build_adjacency(self.num_gates, self.num_nets, ptr(self.net_ptr),
+                                        ptr(self.net_pins), ptr(aptr), ptr(adj), int(adj.shape[0]))
+            if rc < 0:
+                raise FluxionError(int(rc), "fx_build_adjacency failed")
+            self._adj = (aptr, adj[:count])
+        return self._adj
+
+
+@dataclass
+class Placement:
+    """Placement (placement.rs:9-24): SoA f32 positions."""
+    gate_x: np.ndarray
+    gate_y: np.ndarray
+
+    @classmethod
+    def new(cls, num_gates: int) -> "Placement":
+        return cls(np.zeros(num_gates, np.float32), np.zeros(num_gates, np.float32))
+
+    def set_position(self, g: int, x: float, y: float) -> None:
+        self.gate_x[g] = x
+        self.gate_y[g] = y
+
+
+def _leaf_batch(leaf_ptr, leaf_gates, region, circuit: Circuit):
+    aptr, adj = circuit.build_adjacency()
+    keep = (np.ascontiguousarray(leaf_ptr, np.int32), np.ascontiguousarray(leaf_gates, np.int32),
+            np.ascontiguousarray(region, np.float32), aptr,
+            adj if adj.size else np.zeros(1, np.int32))
+    b = FxLeafBatch()
+    b.num_leaves = int(keep[0].shape[0] - 1)
+    b.leaf_ptr = i32p(keep[0])
+    b.leaf_gates = i32p(keep[1])
+    b.region = f32p(keep[2])
+    b.num_gates = circuit.num_gates
+    b.adj_ptr = i32p(keep[3])
+    b.adj = i32p(keep[4])
+    return b, keep
+
+
+def place_leaves(ctx: GpuContext, leaf_ptr, leaf_gates, region, circuit: Circuit,
+                 placement: Placement, config: PlacerConfig, fast: bool = False) -> float:
+    """All leaves of placer.rs:66-69 in one launch; returns the kernel time in ms."""
+    b, keep = _leaf_batch(leaf_ptr, leaf_gates, region, circuit)
+    p = config.to_params()
+    check(lib.fx_place_leaves(ctx.handle, C.byref(b), C.byref(p),
+                              FX_LEAF_FAST if fast else FX_LEAF_EXACT,
+                              ptr(placement.gate_x), ptr(placement.gate_y)))
+    return float(lib.fx_leaves_last_kernel_ms(ctx.handle))
+
+
+def place_leaves_state(ctx: GpuContext, leaf_ptr, leaf_gates, region, circuit: Circuit,
+                       config: PlacerConfig, x, y, vx=None, vy=None, iterations=1, init=False,
+                       fast=False, want_forces=False):
+    """Advance given leaf state by `iterations` (teacher forcing); returns (x, y, vx, vy, fx, fy)."""
+    b, keep = _leaf_batch(leaf_ptr, leaf_gates, region, circuit)
+    p = config.to_params()
+    n = circuit.num_gates
+    x = np.array(x, np.float32, copy=True)
+    y = np.array(y, np.float32, copy=True)
+    vx = np.zeros(n, np.float32) if vx is None else np.array(vx, np.float32, copy=True)
+    vy = np.zeros(n, np.float32) if vy is None else np.array(vy, np.float32, copy=True)
+    fx = np.zeros(n, np.float32) if want_forces else None
+    fy = np.zeros(n, np.float32) if want_forces else None
+    check(lib.fx_place_leaves_state(ctx.handle, C.byref(b), C.byref(p),
+                                    FX_LEAF_FAST if fast else FX_LEAF_EXACT, 1 if init else 0,
+                                    int(iterations), ptr(x), ptr(y), ptr(vx), ptr(vy), ptr(fx),
+                                    ptr(fy)))
+    return x, y, vx, vy, fx, fy
+
+
+class FlatJob:
+    """A device-resident flat placement (dispatch_forces, dispatch.rs:18-106, with state kept in HBM)."""
+
+    def __init__(self, ctx: GpuContext, circuit: Circuit, config: PlacerConfig):
+        self.ctx = ctx
+        self.circuit = circuit
+        self.config = config
+        self._net = circuit.netlist()
+        self._params = config.to_params()
+        h = C.c_void_p()
+        check(lib.fx_job_create(ctx.handle, C.byref(self._net), C.byref(self._params), C.byref(h)))
+        self._h = h
+        self.n = circuit.num_gates
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.fx_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_positions(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        check(lib.fx_job_set_positions(self._h, ptr(x), ptr(y), self.n))
+
+    def positions(self):
+        x = np.zeros(self.n, np.float32)
+        y = np.zeros(self.n, np.float32)
+        check(lib.fx_job_get_positions(self._h, ptr(x), ptr(y), self.n))
+        return x, y
+
+    def set_velocities(self, vx, vy):
+        vx = np.ascontiguousarray(vx, np.float32)
+        vy = np.ascontiguousarray(vy, np.float32)
+        check(lib.fx_job_set_velocities(self._h, ptr(vx), ptr(vy), self.n))
+
+    def velocities(self):
+        vx = np.zeros(self.n, np.float32)
+        vy = np.zeros(self.n, np.float32)
+        check(lib.fx_job_get_velocities(self._h, ptr(vx), ptr(vy), self.n))
+        return vx, vy
+
+    def run(self, iterations: int):
+        check(lib.fx_job_run(self._h, int(iterations)))
+
+    def run_async(self, iterations: int):
+        check(lib.fx_job_run_async(self._h, int(iterations)))
+
+    def sync(self):
+        check(lib.fx_job_sync(self._h))
+
+    def use_graph(self, enable: bool = True):
+        check(lib.fx_job_use_graph(self._h, 1 if enable else 0))
+
+    def forces(self, components: int = 7):
+        fx = np.zeros(self.n, np.float32)
+        fy = np.zeros(self.n, np.float32)
+        check(lib.fx_job_forces(self._h, int(components), ptr(fx), ptr(fy)))
+        return fx, fy
+
+    def density_fields(self):
+        G = self.config.density_grid
+        out = [np.zeros((G, G), np.float32) for _ in range(4)]
+        check(lib.fx_job_density_fields(self._h, *[ptr(a) for a in out]))
+        return tuple(out)  # rho, phi, ex, ey
+
+    def bh_stats(self) -> dict:
+        s = np.zeros(4, np.int64)
+        check(lib.fx_job_bh_stats(self._h, ptr(s)))
+        return {"nodes": int(s[0]), "merge_depth": int(s[1]), "interactions": int(s[2]),
+                "opened": int(s[3])}
+
+    PHASES = ("springs", "bh_keys", "bh_sort", "bh_tree", "bh_walk", "dct", "combine")
+
+    def set_timing(self, phases=()):
+        mask = 0
+        for ph in phases:
+            mask |= 1 << self.PHASES.index(ph)
+        check(lib.fx_job_set_timing(self._h, mask))
+
+    def read_timing(self) -> dict:
+        ms = np.zeros(7, np.float64)
+        cnt = np.zeros(7, np.int64)
+        check(lib.fx_job_read_timing(self._h, ptr(ms), ptr(cnt)))
+        return {ph: (float(ms[k]), int(cnt[k])) for k, ph in enumerate(self.PHASES) if cnt[k]}
+
+    def congestion(self, cols: int, rows: int) -> np.ndarray:
+        out = np.zeros((rows, cols), np.int32)
+        check(lib.fx_congestion(self._h, cols, rows, ptr(out)))
+        return out
+
+    def hpwl(self) -> float:
+        v = C.c_double()
+        check(lib.fx_hpwl(self._h, C.byref(v)))
+        return v.value
+
+    def overflow(self, cols: int, rows: int) -> tuple[float, float]:
+        t, m = C.c_double(), C.c_double()
+        check(lib.fx_overflow(self._h, cols, rows, C.byref(t), C.byref(m)))
+        return t.value, m.value
+
+    def density_overflow(self, bins: int = 50) -> tuple[float, float]:
+        m, o = C.c_double(), C.c_double()
+        check(lib.fx_density_overflow(self._h, bins, C.byref(m), C.byref(o)))
+        return m.value, o.value
+
+
+def dispatch_forces(ctx: GpuContext, circuit: Circuit, placement: Placement,
+                    config: PlacerConfig, iterations: int) -> None:
+    """dispatch_forces (dispatch.rs:18-27): advance the whole placement in place."""
+    with FlatJob(ctx, circuit, config) as job:
+        job.set_positions(placement.gate_x, placement.gate_y)
+        job.run(iterations)
+        placement.gate_x[:], placement.gate_y[:] = job.positions()
+
+
+def dispatch_congestion(ctx: GpuContext, cols: int, rows: int, placement: Placement,
+                        circuit: Circuit) -> np.ndarray:
+    """dispatch_congestion (dispatch.rs:109-180): int32 count grid [rows, cols] (h = v)."""
+    net = circuit.netlist()
+    x = np.ascontiguousarray(placement.gate_x, np.float32)
+    y = np.ascontiguousarray(placement.gate_y, np.float32)
+    out = np.zeros((rows, cols), np.int32)
+    check(lib.fx_dispatch_congestion(ctx.handle, C.byref(net), ptr(x), ptr(y), cols, rows, ptr(out)))
+    return out
+
+
+def fiedler_init(n: int) -> np.ndarray:
+    """partitioner.rs:145-147: v_i = sin(i * 2.71828) in f32 (computed on the host)."""
+    i = np.arange(n, dtype=np.float32)
+    return np.sin(i * np.float32(2.71828)).astype(np.float32)
+
+
+def compute_fiedler_vector(ctx: GpuContext, circuit: Circuit, gate_indices, num_iterations=30,
+                           v0=None, rayleigh=False):
+    """compute_fiedler_vector (partitioner.rs:106-185) on the GPU."""
+    gates = np.ascontiguousarray(gate_indices, np.int32)
+    n = gates.shape[0]
+    aptr, adj = circuit.build_adjacency()
+    adj = adj if adj.size else np.zeros(1, np.int32)
+    v0 = fiedler_init(n) if v0 is None else np.ascontiguousarray(v0, np.float32)
+    out = np.zeros(n, np.float32)
+    rq = C.c_double()
+    check(lib.fx_fiedler(ctx.handle, n, ptr(gates), circuit.num_gates, ptr(aptr), ptr(adj),
+                         ptr(v0), int(num_iterations), ptr(out), C.byref(rq)))
+    return (out, rq.value) if rayleigh else out
+
+
+def split_by_fiedler(ctx: GpuContext, fiedler, gate_indices, balance_factor=0.4):
+    """split_by_fiedler (partitioner.rs:189-213): (left, right) gate lists."""
+    f = np.ascontiguousarray(fiedler, np.float32)
+    g = np.ascontiguousarray(gate_indices, np.int32)
+    out = np.zeros(g.shape[0], np.int32)
+    left = C.c_int32()
+    check(lib.fx_split_by_fiedler(ctx.handle, g.shape[0], ptr(f), ptr(g), float(balance_factor),
+                                  ptr(out), C.byref(left)))
+    return out[:left.value].copy(), out[left.value:].copy()
+
+
+def sort_f32(ctx: GpuContext, keys) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.float32)
+    perm = np.zeros(k.shape[0], np.uint32)
+    check(lib.fx_sort_f32(ctx.handle, k.shape[0], ptr(k), ptr(perm)))
+    return perm
+
+
+def sort_u64(ctx: GpuContext, keys) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.uint64)
+    perm = np.zeros(k.shape[0], np.uint32)
+    check(lib.fx_sort_u64(ctx.handle, k.shape[0], ptr(k), ptr(perm)))
+    return perm
+
+
+class PlacementServer:
+    """Continuous-batching placement server (fx_server_*)."""
+
+    def __init__(self, ctx: GpuContext, max_active: int = 16, hbm_budget: int = 64 << 30,
+                 use_graphs: bool = True):
+        cfg = FxServerConfig(max_active, hbm_budget, 1 if use_graphs else 0)
+        h = C.c_void_p()
+        check(lib.fx_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def submit(self, circuit: Circuit, config: PlacerConfig, x0, y0, iterations: int) -> int:
+        net = circuit.netlist()
+        p = config.to_params()
+        x0 = np.ascontiguousarray(x0, np.float32)
+        y0 = np.ascontiguousarray(y0, np.float32)
+        rid = C.c_int64()
+        check(lib.fx_server_submit(self._h, C.byref(net), C.byref(p), ptr(x0), ptr(y0),
+                                   int(iterations), C.byref(rid)))
+        self._n[rid.value] = circuit.num_gates
+        return rid.value
+
+    def poll(self, rid: int, want_positions=True) -> dict:
+        st = C.c_int32()
+        h, ts, t0, t1 = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        n = self._n[rid]
+        x = np.zeros(n, np.float32) if want_positions else None
+        y = np.zeros(n, np.float32) if want_positions else None
+        check(lib.fx_server_poll(self._h, rid, C.byref(st), ptr(x), ptr(y), C.byref(h),
+                                 C.byref(ts), C.byref(t0), C.byref(t1)))
+        r = {"status": st.value, "t_submit": ts.value, "t_start": t0.value, "t_finish": t1.value}
+        if st.value == 2:
+            r.update(hpwl=h.value, x=x, y=y)
+        return r
+
+    def release(self, rid: int):
+        check(lib.fx_server_release(self._h, rid))
+        self._n.pop(rid, None)
+
+    def drain(self):
+        check(lib.fx_server_drain(self._h))
+
+    def clock(self) -> float:
+        return float(lib.fx_server_clock(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.fx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,994 @@
+// CPU oracle for the HAPR force-directed placement inner loop.
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker: tests, the smoke
+// check and the CPU-baseline leg of the benchmark call it; the product
+// library never links or calls it. Every function restates a reference
+// function operation-for-operation; the citation sits beside each one.
+// Reference files (paths relative to the reference repository root):
+//   HAPR_fluxion_v4/src/algorithm/placer.rs       (leaf force loop)
+//   HAPR_fluxion_v4/src/data/circuit.rs           (adjacency)
+//   HAPR_fluxion_v4/src/algorithm/congestion.rs   (congestion grid)
+//   HAPR_fluxion_v4/src/data/grid.rs              (cell_at, overflow)
+//   HAPR_fluxion_v4/src/data/placement.rs         (HPWL)
+//   HAPR_fluxion_v4/src/algorithm/partitioner.rs  (power iteration)
+//   neon_old_codebase/src/cpp/BarnesHutTree.cpp   (Barnes-Hut)
+//   neon_old_codebase/src/python/fluxion/force_electrostatic.py (DCT Poisson)
+//
+// Build: g++ -O3 -ffp-contract=off (no fast-math), so FP32 results follow
+// IEEE single precision step by step. Templates are instantiated in float
+// (the CPU path of record) and double (the FP64 oracle).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+// Rust f32::max / f64::max: NaN-ignoring maximum (placer.rs:263).
+template <typename T> inline T rust_max(T a, T b) { return std::fmax(a, b); }
+
+// Rust `clamp(min, max)` (placer.rs:278-279): self<min -> min, self>max -> max.
+template <typename T> inline T rust_clamp(T v, T lo, T hi) {
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  return v;
+}
+
+// Rust `(q as usize)`: saturating, NaN -> 0 (grid.rs:58-62).
+inline uint64_t rust_as_usize(float q) {
+  if (!(q > 0.0f)) return 0;
+  if (q >= 18446744073709551616.0f) return UINT64_MAX;
+  return (uint64_t)q;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Circuit::build_adjacency (circuit.rs:154-171).
+ORC_API int64_t orc_build_adjacency(int32_t n, int32_t e, const int32_t* net_ptr,
+                                    const int32_t* net_pins, int32_t* adj_ptr,
+                                    int32_t* adj, int64_t cap) {
+  std::vector<std::vector<int32_t>> a(n);
+  for (int32_t k = 0; k < e; ++k) {
+    for (int32_t i = net_ptr[k]; i < net_ptr[k + 1]; ++i)
+      for (int32_t j = i + 1; j < net_ptr[k + 1]; ++j) {
+        int32_t u = net_pins[i], v = net_pins[j];
+        if (std::find(a[u].begin(), a[u].end(), v) == a[u].end()) a[u].push_back(v);
+        if (std::find(a[v].begin(), a[v].end(), u) == a[v].end()) a[v].push_back(u);
+      }
+  }
+  int64_t total = 0;
+  for (auto& l : a) total += (int64_t)l.size();
+  if (!adj_ptr) return total;
+  if (adj && total > cap) return -2;
+  int64_t o = 0;
+  adj_ptr[0] = 0;
+  for (int32_t g = 0; g < n; ++g) {
+    for (int32_t v : a[g]) {
+      if (adj) adj[o] = v;
+      ++o;
+    }
+    adj_ptr[g + 1] = (int32_t)o;
+  }
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// place_gates_in_leaf (placer.rs:210-287) for one leaf.
+// gates: the leaf's global gate ids in leaf order; adj_ptr/adj: the global
+// Circuit.adjacency CSR; g2l: scratch of size num_gates filled with -1.
+// init!=0 runs the initialisation of placer.rs:226-229 first; otherwise the
+// given x/y/vx/vy state is advanced. fx/fy (optional) receive the force of
+// the last iteration (teacher-forced comparisons).
+template <typename T>
+static void leaf_run(int32_t n, const int32_t* gates, const int32_t* adj_ptr,
+                     const int32_t* adj, int32_t* g2l, T cx, T cy, T w, T h,
+                     T attract, T repel, T damping, T dt, int32_t iters, int32_t init,
+                     T* x, T* y, T* vx, T* vy, T* fx_out, T* fy_out) {
+  if (n == 0) return;
+  const T half_w = w / T(2), half_h = h / T(2);
+  const T min_x = cx - half_w, min_y = cy - half_h;
+  if (init) {
+    for (int32_t i = 0; i < n; ++i) {
+      x[i] = min_x + (T(i) / T(n)) * w;
+      y[i] = min_y + (T((int64_t)i * 7 % n) / T(n)) * h;
+      vx[i] = T(0);
+      vy[i] = T(0);
+    }
+  }
+  for (int32_t i = 0; i < n; ++i) g2l[gates[i]] = i;
+  std::vector<T> fx(n), fy(n);
+  for (int32_t it = 0; it < iters; ++it) {
+    std::fill(fx.begin(), fx.end(), T(0));
+    std::fill(fy.begin(), fy.end(), T(0));
+    // Attraction (placer.rs:242-255).
+    for (int32_t li = 0; li < n; ++li) {
+      const int32_t g = gates[li];
+      for (int32_t k = adj_ptr[g]; k < adj_ptr[g + 1]; ++k) {
+        const int32_t lj = g2l[adj[k]];
+        if (lj < 0) continue;
+        const T dx = x[lj] - x[li];
+        const T dy = y[lj] - y[li];
+        fx[li] += attract * dx;
+        fy[li] += attract * dy;
+      }
+    }
+    // Exact pairwise repulsion (placer.rs:257-272).
+    for (int32_t i = 0; i < n; ++i)
+      for (int32_t j = i + 1; j < n; ++j) {
+        const T dx = x[i] - x[j];
+        const T dy = y[i] - y[j];
+        const T dist_sq = dx * dx + dy * dy;
+        const T dist = rust_max(std::sqrt(dist_sq), T(0.1));
+        const T force = repel / (dist * dist);
+        const T fx_ij = force * dx / dist;
+        const T fy_ij = force * dy / dist;
+        fx[i] += fx_ij;
+        fy[i] += fy_ij;
+        fx[j] -= fx_ij;
+        fy[j] -= fy_ij;
+      }
+    // Damped Euler step with clamp (placer.rs:274-280).
+    for (int32_t i = 0; i < n; ++i) {
+      vx[i] = (vx[i] + fx[i] * dt) * damping;
+      vy[i] = (vy[i] + fy[i] * dt) * damping;
+      x[i] = rust_clamp(x[i] + vx[i] * dt, min_x, min_x + w);
+      y[i] = rust_clamp(y[i] + vy[i] * dt, min_y, min_y + h);
+    }
+  }
+  if (fx_out)
+    for (int32_t i = 0; i < n; ++i) {
+      fx_out[i] = fx[i];
+      fy_out[i] = fy[i];
+    }
+  for (int32_t i = 0; i < n; ++i) g2l[gates[i]] = -1;
+}
+
+// All leaves of a batch (the loop of placer.rs:66-69), leaves independent.
+// region: cx,cy,w,h per leaf. x/y: global positions, written per leaf.
+// vx/vy/fx/fy optional (global arrays). nthreads>1 parallelises over leaves.
+template <typename T>
+static void leaves_run(int32_t num_leaves, const int32_t* leaf_ptr,
+                       const int32_t* leaf_gates, const float* region, int32_t num_gates,
+                       const int32_t* adj_ptr, const int32_t* adj, float attract,
+                       float repel, float damping, float dt, int32_t iters, int32_t init,
+                       T* x, T* y, T* vx, T* vy, T* fx, T* fy, int32_t nthreads) {
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+    std::vector<int32_t> g2l(num_gates, -1);
+    std::vector<T> lx, ly, lvx, lvy, lfx, lfy;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int32_t c = 0; c < num_leaves; ++c) {
+      const int32_t s = leaf_ptr[c], n = leaf_ptr[c + 1] - s;
+      const int32_t* g = leaf_gates + s;
+      lx.assign(n, T(0));
+      ly.assign(n, T(0));
+      lvx.assign(n, T(0));
+      lvy.assign(n, T(0));
+      lfx.assign(n, T(0));
+      lfy.assign(n, T(0));
+      if (!init)
+        for (int32_t i = 0; i < n; ++i) {
+          lx[i] = x[g[i]];
+          ly[i] = y[g[i]];
+          if (vx) {
+            lvx[i] = vx[g[i]];
+            lvy[i] = vy[g[i]];
+          }
+        }
+      leaf_run<T>(n, g, adj_ptr, adj, g2l.data(), T(region[4 * c]), T(region[4 * c + 1]),
+                  T(region[4 * c + 2]), T(region[4 * c + 3]), T(attract), T(repel),
+                  T(damping), T(dt), iters, init, lx.data(), ly.data(), lvx.data(),
+                  lvy.data(), lfx.data(), lfy.data());
+      for (int32_t i = 0; i < n; ++i) {
+        x[g[i]] = lx[i];
+        y[g[i]] = ly[i];
+        if (vx) {
+          vx[g[i]] = lvx[i];
+          vy[g[i]] = lvy[i];
+        }
+        if (fx) {
+          fx[g[i]] = lfx[i];
+          fy[g[i]] = lfy[i];
+        }
+      }
+    }
+  }
+}
+
+ORC_API void orc_place_leaves_f32(int32_t num_leaves, const int32_t* leaf_ptr,
+                                  const int32_t* leaf_gates, const float* region,
+                                  int32_t num_gates, const int32_t* adj_ptr,
+                                  const int32_t* adj, float attract, float repel,
+                                  float damping, float dt, int32_t iters, int32_t init,
+                                  float* x, float* y, float* vx, float* vy, float* fx,
+                                  float* fy, int32_t nthreads) {
+  leaves_run<float>(num_leaves, leaf_ptr, leaf_gates, region, num_gates, adj_ptr, adj,
+                    attract, repel, damping, dt, iters, init, x, y, vx, vy, fx, fy,
+                    nthreads);
+}
+
+ORC_API void orc_place_leaves_f64(int32_t num_leaves, const int32_t* leaf_ptr,
+                                  const int32_t* leaf_gates, const float* region,
+                                  int32_t num_gates, const int32_t* adj_ptr,
+                                  const int32_t* adj, float attract, float repel,
+                                  float damping, float dt, int32_t iters, int32_t init,
+                                  double* x, double* y, double* vx, double* vy,
+                                  double* fx, double* fy, int32_t nthreads) {
+  leaves_run<double>(num_leaves, leaf_ptr, leaf_gates, region, num_gates, adj_ptr, adj,
+                     attract, repel, damping, dt, iters, init, x, y, vx, vy, fx, fy,
+                     nthreads);
+}
+
+// ---------------------------------------------------------------------------
+// compute_congestion_cpu (congestion.rs:11-62) with RoutingGrid::cell_at
+// (grid.rs:58-62). Counts are accumulated in f32 as the reference does and
+// returned as int32 (exact below 2^24). Returns 0, or 1 if h != v anywhere.
+ORC_API int32_t orc_congestion(int32_t num_nets, const int32_t* net_ptr,
+                               const int32_t* net_pins, const float* x, const float* y,
+                               float chip_w, float chip_h, int32_t cols, int32_t rows,
+                               int32_t* counts) {
+  const float cw = chip_w / (float)cols;
+  const float ch = chip_h / (float)rows;
+  const size_t cells = (size_t)cols * rows;
+  std::vector<float> hc(cells, 0.0f), vc(cells, 0.0f);
+  auto cell_at = [&](float px, float py, uint64_t& c, uint64_t& r) {
+    c = std::min<uint64_t>(rust_as_usize(px / cw), (uint64_t)cols - 1);
+    r = std::min<uint64_t>(rust_as_usize(py / ch), (uint64_t)rows - 1);
+  };
+  for (int32_t e = 0; e < num_nets; ++e) {
+    const int32_t s = net_ptr[e], t = net_ptr[e + 1];
+    if (t - s < 2) continue;
+    float min_x = std::numeric_limits<float>::max(), max_x = std::numeric_limits<float>::lowest();
+    float min_y = min_x, max_y = max_x;
+    for (int32_t k = s; k < t; ++k) {
+      const float px = x[net_pins[k]], py = y[net_pins[k]];
+      min_x = std::fmin(min_x, px);
+      max_x = std::fmax(max_x, px);
+      min_y = std::fmin(min_y, py);
+      max_y = std::fmax(max_y, py);
+    }
+    uint64_t c0, r0, c1, r1;
+    cell_at(min_x, min_y, c0, r0);
+    cell_at(max_x, max_y, c1, r1);
+    for (uint64_t r = r0; r <= r1; ++r)
+      for (uint64_t c = c0; c <= c1; ++c) {
+        const uint64_t idx = r * cols + c;
+        if (idx < cells) hc[idx] += 1.0f;
+      }
+    for (uint64_t c = c0; c <= c1; ++c)
+      for (uint64_t r = r0; r <= r1; ++r) {
+        const uint64_t idx = r * cols + c;
+        if (idx < cells) vc[idx] += 1.0f;
+      }
+  }
+  int32_t mismatch = 0;
+  for (size_t i = 0; i < cells; ++i) {
+    counts[i] = (int32_t)hc[i];
+    if (hc[i] != vc[i]) mismatch = 1;
+  }
+  return mismatch;
+}
+
+// RoutingGrid::max_overflow / total_overflow (grid.rs:70-91) on h = v = counts,
+// capacity (cw+ch)*0.5 (grid.rs:31), in f32 as the reference and in f64.
+ORC_API void orc_overflow_f32(const int32_t* counts, int32_t cols, int32_t rows,
+                              float chip_w, float chip_h, float* total, float* maxo) {
+  const float cw = chip_w / (float)cols, ch = chip_h / (float)rows;
+  const float cap = (cw + ch) * 0.5f;
+  float t = 0.0f, m = 0.0f;
+  for (int32_t r = 0; r < rows; ++r)
+    for (int32_t c = 0; c < cols; ++c) {
+      const float h = (float)counts[(size_t)r * cols + c];
+      const float o = std::fmax((h + h) - cap, 0.0f);
+      t += o;
+      m = std::fmax(m, o);
+    }
+  *total = t;
+  *maxo = m;
+}
+
+ORC_API void orc_overflow_f64(const int32_t* counts, int32_t cols, int32_t rows,
+                              float chip_w, float chip_h, double* total, double* maxo) {
+  const float cw = chip_w / (float)cols, ch = chip_h / (float)rows;
+  const double cap = (double)((cw + ch) * 0.5f);
+  double t = 0.0, m = 0.0;
+  for (int32_t r = 0; r < rows; ++r)
+    for (int32_t c = 0; c < cols; ++c) {
+      const double h = (double)counts[(size_t)r * cols + c];
+      const double o = std::fmax((h + h) - cap, 0.0);
+      t += o;
+      m = std::fmax(m, o);
+    }
+  *total = t;
+  *maxo = m;
+}
+
+// Placement::compute_hpwl (placement.rs:48-79): f32 accumulation as the
+// reference, and an FP64 oracle (all arithmetic in double).
+ORC_API float orc_hpwl_f32(int32_t num_nets, const int32_t* net_ptr,
+                           const int32_t* net_pins, const float* x, const float* y) {
+  float total = 0.0f;
+  for (int32_t e = 0; e < num_nets; ++e) {
+    const int32_t s = net_ptr[e], t = net_ptr[e + 1];
+    if (t - s < 2) continue;
+    float min_x = std::numeric_limits<float>::max(), max_x = std::numeric_limits<float>::lowest();
+    float min_y = min_x, max_y = max_x;
+    for (int32_t k = s; k < t; ++k) {
+      min_x = std::fmin(min_x, x[net_pins[k]]);
+      max_x = std::fmax(max_x, x[net_pins[k]]);
+      min_y = std::fmin(min_y, y[net_pins[k]]);
+      max_y = std::fmax(max_y, y[net_pins[k]]);
+    }
+    total += (max_x - min_x) + (max_y - min_y);
+  }
+  return total;
+}
+
+ORC_API double orc_hpwl_f64(int32_t num_nets, const int32_t* net_ptr,
+                            const int32_t* net_pins, const double* x, const double* y) {
+  double total = 0.0;
+  for (int32_t e = 0; e < num_nets; ++e) {
+    const int32_t s = net_ptr[e], t = net_ptr[e + 1];
+    if (t - s < 2) continue;
+    double min_x = x[net_pins[s]], max_x = min_x, min_y = y[net_pins[s]], max_y = min_y;
+    for (int32_t k = s + 1; k < t; ++k) {
+      min_x = std::fmin(min_x, x[net_pins[k]]);
+      max_x = std::fmax(max_x, x[net_pins[k]]);
+      min_y = std::fmin(min_y, y[net_pins[k]]);
+      max_y = std::fmax(max_y, y[net_pins[k]]);
+    }
+    total += (max_x - min_x) + (max_y - min_y);
+  }
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// Net springs for flat netlists. Per net e with k>=2 pins and centroid c_e
+// (pin-order sum / k), every pin's gate i receives w*k*(c_e - x_i): the
+// clique force sum_{j in e, j!=i} w*(x_j - x_i) of placer.rs:248-251 written
+// through the centroid, which is also the star force with the
+// clique-equivalent weight k*w. A gate sums its incident nets in ascending
+// net id order.
+template <typename T>
+static void springs(int32_t n, int32_t e, const int32_t* net_ptr, const int32_t* net_pins,
+                    const T* x, const T* y, T w, T* fx, T* fy) {
+  std::vector<T> cxn(e), cyn(e), wk(e);
+  for (int32_t k = 0; k < e; ++k) {
+    const int32_t s = net_ptr[k], t = net_ptr[k + 1];
+    T sx = T(0), sy = T(0);
+    for (int32_t p = s; p < t; ++p) {
+      sx += x[net_pins[p]];
+      sy += y[net_pins[p]];
+    }
+    const T deg = T(t - s);
+    cxn[k] = sx / deg;
+    cyn[k] = sy / deg;
+    wk[k] = (t - s >= 2) ? w * deg : T(0);
+  }
+  // Gate -> incident nets, ascending net id (stable counting sort).
+  std::vector<int32_t> gptr(n + 1, 0), gnet(net_ptr[e]);
+  for (int32_t p = 0; p < net_ptr[e]; ++p) gptr[net_pins[p] + 1]++;
+  for (int32_t g = 0; g < n; ++g) gptr[g + 1] += gptr[g];
+  std::vector<int32_t> fill(gptr.begin(), gptr.end() - 1);
+  for (int32_t k = 0; k < e; ++k)
+    for (int32_t p = net_ptr[k]; p < net_ptr[k + 1]; ++p) gnet[fill[net_pins[p]]++] = k;
+  for (int32_t g = 0; g < n; ++g) {
+    T ax = T(0), ay = T(0);
+    for (int32_t q = gptr[g]; q < gptr[g + 1]; ++q) {
+      const int32_t k = gnet[q];
+      ax += wk[k] * (cxn[k] - x[g]);
+      ay += wk[k] * (cyn[k] - y[g]);
+    }
+    fx[g] = ax;
+    fy[g] = ay;
+  }
+}
+
+ORC_API void orc_springs_f32(int32_t n, int32_t e, const int32_t* net_ptr,
+                             const int32_t* net_pins, const float* x, const float* y,
+                             float w, float* fx, float* fy) {
+  springs<float>(n, e, net_ptr, net_pins, x, y, w, fx, fy);
+}
+ORC_API void orc_springs_f64(int32_t n, int32_t e, const int32_t* net_ptr,
+                             const int32_t* net_pins, const double* x, const double* y,
+                             double w, double* fx, double* fy) {
+  springs<double>(n, e, net_ptr, net_pins, x, y, w, fx, fy);
+}
+
+// ---------------------------------------------------------------------------
+// Barnes-Hut repulsion.
+//
+// (1) bh_literal: BarnesHutTree.cpp:16-168 in double, as written: root box
+//     = bbox +- 1.0 squared about its centre (24-46); one particle per leaf,
+//     co-located merge when size < 1e-8 (64-72); quadrant rule px<=mx -> W,
+//     py<=my -> S (107-123); incremental charge-weighted centre (84-96);
+//     force walk skipping the leaf that holds p_idx (149), accept leaf or
+//     size/dist < theta, F = k q Q / d^2 along (p-c)/d with d >= min_dist.
+// (2) bh_record: the FP32 path of record. Same insertion tree and walk, with
+//     three stated changes: the cell geometry stays in double (FP32
+//     midpoints stop halving below one ulp), the co-location merge happens at
+//     depth D = min(32, first depth whose cell size < 1e-8) (the same rule
+//     whenever that depth is <= 32), and a node's centre is sum(q x)/sum(q)
+//     computed from exact 64-bit fixed-point sums (equal in exact arithmetic
+//     to the incremental update). Distances, forces and the theta test are
+//     FP32. Charges are 1 (unit gates).
+
+namespace {
+
+struct LitNode {
+  double x_min, y_min, x_max, y_max;
+  double cx = 0, cy = 0, total_charge = 0;
+  int count = 0, particle_idx = -1;
+  int child[4] = {-1, -1, -1, -1};
+  bool leaf() const { return child[0] < 0 && child[1] < 0 && child[2] < 0 && child[3] < 0; }
+  double size() const { return std::max(x_max - x_min, y_max - y_min); }
+};
+
+struct LitTree {
+  std::vector<LitNode> nodes;
+  const double *px, *py, *pq;
+  double theta;
+  int64_t interactions = 0, opened = 0;
+
+  void build(const double* x, const double* y, const double* q, int n) {
+    px = x; py = y; pq = q;
+    nodes.clear();
+    if (n == 0) return;
+    double min_x = x[0], max_x = x[0], min_y = y[0], max_y = y[0];
+    for (int i = 1; i < n; ++i) {
+      if (x[i] < min_x) min_x = x[i];
+      if (x[i] > max_x) max_x = x[i];
+      if (y[i] < min_y) min_y = y[i];
+      if (y[i] > max_y) max_y = y[i];
+    }
+    const double margin = 1.0;
+    min_x -= margin; max_x += margin; min_y -= margin; max_y += margin;
+    const double size = std::max(max_x - min_x, max_y - min_y);
+    const double cx = (min_x + max_x) / 2.0, cy = (min_y + max_y) / 2.0;
+    LitNode root;
+    root.x_min = cx - size / 2; root.x_max = cx + size / 2;
+    root.y_min = cy - size / 2; root.y_max = cy + size / 2;
+    nodes.push_back(root);
+    for (int i = 0; i < n; ++i) insert(0, i, x[i], y[i], q[i]);
+  }
+
+  void insert(int ni, int idx, double x, double y, double c) {
+    if (nodes[ni].count == 0) {
+      LitNode& nd = nodes[ni];
+      nd.particle_idx = idx; nd.cx = x; nd.cy = y; nd.total_charge = c; nd.count = 1;
+      return;
+    }
+    if (nodes[ni].leaf()) {
+      if (nodes[ni].size() < 1e-8) {
+        LitNode& nd = nodes[ni];
+        const double tc = nd.total_charge + c;
+        nd.cx = (nd.cx * nd.total_charge + x * c) / tc;
+        nd.cy = (nd.cy * nd.total_charge + y * c) / tc;
+        nd.total_charge = tc;
+        nd.count++;
+        return;
+      }
+      const int old = nodes[ni].particle_idx;
+      const double ox = px[old], oy = py[old], oc = pq[old];
+      nodes[ni].particle_idx = -1;
+      insert_child(ni, old, ox, oy, oc);
+      insert_child(ni, idx, x, y, c);
+      LitNode& nd = nodes[ni];
+      const double tc = oc + c;
+      nd.cx = (ox * oc + x * c) / tc;
+      nd.cy = (oy * oc + y * c) / tc;
+      nd.total_charge = tc;
+      nd.count = 2;
+    } else {
+      insert_child(ni, idx, x, y, c);
+      LitNode& nd = nodes[ni];
+      const double tc = nd.total_charge + c;
+      nd.cx = (nd.cx * nd.total_charge + x * c) / tc;
+      nd.cy = (nd.cy * nd.total_charge + y * c) / tc;
+      nd.total_charge = tc;
+      nd.count++;
+    }
+  }
+
+  void insert_child(int ni, int idx, double x, double y, double c) {
+    const LitNode nd = nodes[ni];
+    const double mx = (nd.x_min + nd.x_max) / 2, my = (nd.y_min + nd.y_max) / 2;
+    int quad;
+    double b[4];
+    if (x <= mx) {
+      if (y <= my) { quad = 2; b[0] = nd.x_min; b[1] = nd.y_min; b[2] = mx; b[3] = my; }
+      else { quad = 0; b[0] = nd.x_min; b[1] = my; b[2] = mx; b[3] = nd.y_max; }
+    } else {
+      if (y <= my) { quad = 3; b[0] = mx; b[1] = nd.y_min; b[2] = nd.x_max; b[3] = my; }
+      else { quad = 1; b[0] = mx; b[1] = my; b[2] = nd.x_max; b[3] = nd.y_max; }
+    }
+    if (nodes[ni].child[quad] < 0) {
+      LitNode ch;
+      ch.x_min = b[0]; ch.y_min = b[1]; ch.x_max = b[2]; ch.y_max = b[3];
+      nodes.push_back(ch);
+      nodes[ni].child[quad] = (int)nodes.size() - 1;
+    }
+    insert(nodes[ni].child[quad], idx, x, y, c);
+  }
+
+  void force(int ni, int p, double x, double y, double c, double k, double md, double& fx,
+             double& fy) {
+    const LitNode& nd = nodes[ni];
+    if (nd.count == 0) return;
+    if (nd.leaf() && nd.particle_idx == p) return;
+    const double dx = x - nd.cx, dy = y - nd.cy;
+    const double dist_sq = dx * dx + dy * dy;
+    double dist = std::sqrt(dist_sq);
+    if (dist < md) dist = md;
+    if (nd.leaf() || (nd.size() / dist < theta)) {
+      const double f = (k * c * nd.total_charge) / (dist * dist);
+      fx += f * (dx / dist);
+      fy += f * (dy / dist);
+      ++interactions;
+    } else {
+      ++opened;
+      for (int i = 0; i < 4; ++i)
+        if (nd.child[i] >= 0) force(nd.child[i], p, x, y, c, k, md, fx, fy);
+    }
+  }
+};
+
+// Path-of-record tree (FP64 geometry, depth-capped merge, fixed-point centres).
+struct RecNode {
+  double x_min, y_min, x_max, y_max;
+  int64_t sx = 0, sy = 0;
+  int count = 0, first = -1, depth = 0;
+  int child[4] = {-1, -1, -1, -1};
+  float cx = 0, cy = 0, q = 0, size = 0;
+  bool leaf() const { return child[0] < 0 && child[1] < 0 && child[2] < 0 && child[3] < 0; }
+};
+
+}  // namespace
+
+// Geometry shared by the path of record and the GPU: root box, merge depth
+// and fixed-point scale. Exposed so tests can check the GPU's values.
+ORC_API void orc_bh_geometry(int32_t n, const float* x, const float* y, double* root,
+                             int32_t* merge_depth, int32_t* scale_bits) {
+  float mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
+  for (int32_t i = 1; i < n; ++i) {
+    mnx = std::fmin(mnx, x[i]); mxx = std::fmax(mxx, x[i]);
+    mny = std::fmin(mny, y[i]); mxy = std::fmax(mxy, y[i]);
+  }
+  double min_x = (double)mnx - 1.0, max_x = (double)mxx + 1.0;
+  double min_y = (double)mny - 1.0, max_y = (double)mxy + 1.0;
+  const double size = std::max(max_x - min_x, max_y - min_y);
+  const double cx = (min_x + max_x) / 2.0, cy = (min_y + max_y) / 2.0;
+  root[0] = cx - size / 2; root[1] = cy - size / 2;
+  root[2] = cx + size / 2; root[3] = cy + size / 2;
+  double xl = root[0], yl = root[1], xh = root[2], yh = root[3];
+  int d = 0;
+  while (d < 32 && std::max(xh - xl, yh - yl) >= 1e-8) {
+    xh = (xl + xh) / 2; yh = (yl + yh) / 2;
+    ++d;
+  }
+  *merge_depth = d;
+  const double s_root = std::max(root[2] - root[0], root[3] - root[1]);
+  const uint64_t n_bits = 64 - __builtin_clzll((uint64_t)(n > 0 ? n : 1));
+  const uint64_t s_int = (uint64_t)std::ceil(s_root);
+  const uint64_t s_bits = 64 - __builtin_clzll(s_int > 0 ? s_int : 1);
+  int sb = 62 - (int)n_bits - (int)s_bits;
+  *scale_bits = std::max(0, std::min(40, sb));
+}
+
+ORC_API void orc_bh_record_f32(int32_t n, const float* x, const float* y, float k,
+                               float theta, float min_dist, float* fx, float* fy,
+                               int64_t* stats, int32_t nthreads) {
+  if (n <= 0) return;
+  double root[4];
+  int32_t D, sbits;
+  orc_bh_geometry(n, x, y, root, &D, &sbits);
+  const double scale = std::ldexp(1.0, sbits);
+  std::vector<RecNode> nodes;
+  nodes.reserve(2 * (size_t)n + 16);
+  RecNode r;
+  r.x_min = root[0]; r.y_min = root[1]; r.x_max = root[2]; r.y_max = root[3];
+  nodes.push_back(r);
+  std::vector<int64_t> X(n), Y(n);
+  for (int32_t i = 0; i < n; ++i) {
+    X[i] = std::llrint(((double)x[i] - root[0]) * scale);
+    Y[i] = std::llrint(((double)y[i] - root[1]) * scale);
+  }
+  // Iterative form of insert/insertIntoChild (BarnesHutTree.cpp:53-134).
+  auto child_of = [&](int ni, int idx) {
+    const RecNode nd = nodes[ni];
+    const double mx = (nd.x_min + nd.x_max) / 2, my = (nd.y_min + nd.y_max) / 2;
+    const double px = (double)x[idx], py = (double)y[idx];
+    int quad;
+    double b[4];
+    if (px <= mx) {
+      if (py <= my) { quad = 2; b[0] = nd.x_min; b[1] = nd.y_min; b[2] = mx; b[3] = my; }
+      else { quad = 0; b[0] = nd.x_min; b[1] = my; b[2] = mx; b[3] = nd.y_max; }
+    } else {
+      if (py <= my) { quad = 3; b[0] = mx; b[1] = nd.y_min; b[2] = nd.x_max; b[3] = my; }
+      else { quad = 1; b[0] = mx; b[1] = my; b[2] = nd.x_max; b[3] = nd.y_max; }
+    }
+    if (nodes[ni].child[quad] < 0) {
+      RecNode ch;
+      ch.x_min = b[0]; ch.y_min = b[1]; ch.x_max = b[2]; ch.y_max = b[3];
+      ch.depth = nd.depth + 1;
+      nodes.push_back(ch);
+      nodes[ni].child[quad] = (int)nodes.size() - 1;
+    }
+    return nodes[ni].child[quad];
+  };
+  for (int32_t idx = 0; idx < n; ++idx) {
+    int ni = 0;
+    for (;;) {
+      RecNode& nd = nodes[ni];
+      if (nd.count == 0) {
+        nd.first = idx; nd.count = 1; nd.sx = X[idx]; nd.sy = Y[idx];
+        break;
+      }
+      if (nd.leaf()) {
+        if (nd.depth >= D) {  // co-located merge
+          nd.count++; nd.sx += X[idx]; nd.sy += Y[idx];
+          break;
+        }
+        const int old = nd.first;
+        nd.first = -1;
+        nd.count = 2;
+        nd.sx += X[idx]; nd.sy += Y[idx];
+        // Push the old particle one level down; it lands in an empty child.
+        const int oc = child_of(ni, old);
+        RecNode& och = nodes[oc];
+        och.first = old; och.count = 1; och.sx = X[old]; och.sy = Y[old];
+        ni = child_of(ni, idx);
+        continue;
+      }
+      nd.count++; nd.sx += X[idx]; nd.sy += Y[idx];
+      ni = child_of(ni, idx);
+    }
+  }
+  for (RecNode& nd : nodes) {
+    nd.cx = (float)(root[0] + ((double)nd.sx / nd.count) / scale);
+    nd.cy = (float)(root[1] + ((double)nd.sy / nd.count) / scale);
+    nd.q = (float)nd.count;
+    nd.size = (float)std::max(nd.x_max - nd.x_min, nd.y_max - nd.y_min);
+  }
+  int64_t inter = 0, opened = 0;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256) reduction(+ : inter, opened)
+#endif
+  for (int32_t p = 0; p < n; ++p) {
+    const float px = x[p], py = y[p], charge = 1.0f;
+    float ax = 0.0f, ay = 0.0f;
+    int stack[4 * 64 + 8];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+      const RecNode& nd = nodes[stack[--sp]];
+      if (nd.count == 0) continue;
+      const bool lf = nd.leaf();
+      if (lf && nd.first == p) continue;
+      const float dx = px - nd.cx, dy = py - nd.cy;
+      const float dist_sq = dx * dx + dy * dy;
+      float dist = std::sqrt(dist_sq);
+      if (dist < min_dist) dist = min_dist;
+      if (lf || (nd.size / dist < theta)) {
+        const float f = (k * charge * nd.q) / (dist * dist);
+        ax += f * (dx / dist);
+        ay += f * (dy / dist);
+        ++inter;
+      } else {
+        ++opened;
+        for (int c = 3; c >= 0; --c)
+          if (nd.child[c] >= 0) stack[sp++] = nd.child[c];
+      }
+    }
+    fx[p] = ax;
+    fy[p] = ay;
+  }
+  if (stats) {
+    stats[0] = inter;
+    stats[1] = opened;
+    stats[2] = (int64_t)nodes.size();
+    stats[3] = D;
+  }
+}
+
+ORC_API void orc_bh_literal_f64(int32_t n, const double* x, const double* y,
+                                const double* charges, double k, double theta,
+                                double min_dist, double* fx, double* fy, int64_t* stats) {
+  LitTree t;
+  t.theta = theta;
+  t.build(x, y, charges, n);
+  for (int32_t p = 0; p < n; ++p) {
+    double ax = 0, ay = 0;
+    if (!t.nodes.empty()) t.force(0, p, x[p], y[p], charges[p], k, min_dist, ax, ay);
+    fx[p] = ax;
+    fy[p] = ay;
+  }
+  if (stats) {
+    stats[0] = t.interactions;
+    stats[1] = t.opened;
+    stats[2] = (int64_t)t.nodes.size();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ElectrostaticSmoothingForce (force_electrostatic.py:47-157) in double:
+// point-in-bin density (47-60), minus target (122-123), DCT-II ortho (69),
+// eigenvalues (w cos(pi k/n) - w)^2 with lambda_00 = 1 (72-82), phi_hat =
+// -rho_hat/lambda with phi_hat_00 = 0 (85-86), inverse DCT (89), centred
+// differences with one-sided edges (93-103), nearest-bin gather q*E (134-145)
+// times the weight (153). Charges q = area = 1. The 2-D transforms are the
+// matrix products C X C^T and C^T X C with the orthonormal DCT-II matrix C.
+// Outputs rho (density), phi, ex, ey: G*G row-major [row][col], optional.
+ORC_API void orc_density_f64(int32_t n, const float* x, const float* y, float die_w,
+                             float die_h, int32_t G, double target, double weight,
+                             double* fx, double* fy, double* rho_out, double* phi_out,
+                             double* ex_out, double* ey_out) {
+  const size_t GG = (size_t)G * G;
+  const double bin_w = (double)die_w / G, bin_h = (double)die_h / G;
+  std::vector<double> rho(GG, 0.0);
+  std::vector<int32_t> bc(n), br(n);
+  for (int32_t i = 0; i < n; ++i) {
+    double qc = std::min(std::max((double)x[i] / bin_w, 0.0), (double)(G - 1));
+    double qr = std::min(std::max((double)y[i] / bin_h, 0.0), (double)(G - 1));
+    bc[i] = (int32_t)qc;
+    br[i] = (int32_t)qr;
+    rho[(size_t)br[i] * G + bc[i]] += 1.0 / (bin_w * bin_h);
+  }
+  if (rho_out) std::memcpy(rho_out, rho.data(), GG * sizeof(double));
+  std::vector<double> C(GG), t(GG), a(GG);
+  const double pi = 3.14159265358979323846;
+  for (int32_t kk = 0; kk < G; ++kk) {
+    const double s = kk == 0 ? std::sqrt(1.0 / G) : std::sqrt(2.0 / G);
+    for (int32_t m = 0; m < G; ++m) C[(size_t)kk * G + m] = s * std::cos(pi * kk * (2 * m + 1) / (2.0 * G));
+  }
+  for (size_t i = 0; i < GG; ++i) a[i] = rho[i] - target;
+  // t = C * a ; rho_hat = t * C^T
+  auto matmul = [&](const std::vector<double>& A, bool ta, const std::vector<double>& B,
+                    std::vector<double>& out) {
+    std::vector<double> o(GG, 0.0);
+    for (int32_t i = 0; i < G; ++i)
+      for (int32_t k2 = 0; k2 < G; ++k2) {
+        const double av = ta ? A[(size_t)k2 * G + i] : A[(size_t)i * G + k2];
+        if (av == 0.0) continue;
+        double* orow = &o[(size_t)i * G];
+        const double* brow = &B[(size_t)k2 * G];
+        for (int32_t j = 0; j < G; ++j) orow[j] += av * brow[j];
+      }
+    out.swap(o);
+  };
+  // Products with a transposed right operand go through an explicit C^T.
+  std::vector<double> Ct(GG);
+  for (int32_t i = 0; i < G; ++i)
+    for (int32_t j = 0; j < G; ++j) Ct[(size_t)j * G + i] = C[(size_t)i * G + j];
+  matmul(C, false, a, t);
+  std::vector<double> rh;
+  matmul(t, false, Ct, rh);
+  const double wx = 1.0 / bin_w, wy = 1.0 / bin_h;
+  for (int32_t ky = 0; ky < G; ++ky)
+    for (int32_t kx = 0; kx < G; ++kx) {
+      const double ex = wx * std::cos(pi * kx / G) - wx;
+      const double ey = wy * std::cos(pi * ky / G) - wy;
+      double ev = ex * ex + ey * ey;
+      if (kx == 0 && ky == 0) ev = 1.0;
+      double ph = -rh[(size_t)ky * G + kx] / ev;
+      if (kx == 0 && ky == 0) ph = 0.0;
+      rh[(size_t)ky * G + kx] = ph;
+    }
+  // phi = C^T * phi_hat * C
+  matmul(C, true, rh, t);
+  std::vector<double> phi;
+  matmul(t, false, C, phi);
+  if (phi_out) std::memcpy(phi_out, phi.data(), GG * sizeof(double));
+  std::vector<double> Ex(GG, 0.0), Ey(GG, 0.0);
+  for (int32_t r = 0; r < G; ++r)
+    for (int32_t c = 0; c < G; ++c) {
+      const size_t i = (size_t)r * G + c;
+      if (c == 0) Ex[i] = -(phi[i + 1] - phi[i]) * wx;
+      else if (c == G - 1) Ex[i] = -(phi[i] - phi[i - 1]) * wx;
+      else Ex[i] = -(phi[i + 1] - phi[i - 1]) * wx * 0.5;
+      if (r == 0) Ey[i] = -(phi[i + G] - phi[i]) * wy;
+      else if (r == G - 1) Ey[i] = -(phi[i] - phi[i - G]) * wy;
+      else Ey[i] = -(phi[i + G] - phi[i - G]) * wy * 0.5;
+    }
+  if (ex_out) std::memcpy(ex_out, Ex.data(), GG * sizeof(double));
+  if (ey_out) std::memcpy(ey_out, Ey.data(), GG * sizeof(double));
+  if (fx)
+    for (int32_t i = 0; i < n; ++i) {
+      const size_t b = (size_t)br[i] * G + bc[i];
+      fx[i] = (1.0 * Ex[b]) * weight;
+      fy[i] = (1.0 * Ey[b]) * weight;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Flat global placement step: springs + Barnes-Hut + DCT density, then the
+// damped Euler step of placer.rs:274-280 clamped to the die. FP32 path of
+// record (density evaluated in double, added in FP32). components: bit 0
+// springs, bit 1 BH, bit 2 density. If iters==0, only the forces at the given
+// positions are written (teacher forcing).
+ORC_API void orc_flat_run_f32(int32_t n, int32_t e, const int32_t* net_ptr,
+                              const int32_t* net_pins, float die_w, float die_h,
+                              float attract, float repel, float damping, float dt,
+                              float theta, float min_dist, float dens_weight,
+                              float target, int32_t G, int32_t components, int32_t iters,
+                              float* x, float* y, float* vx, float* vy, float* fx,
+                              float* fy, int32_t nthreads) {
+  std::vector<float> sfx(n), sfy(n), bfx(n), bfy(n);
+  std::vector<double> dfx(n), dfy(n);
+  const int32_t steps = iters > 0 ? iters : 1;
+  for (int32_t it = 0; it < steps; ++it) {
+    std::fill(sfx.begin(), sfx.end(), 0.0f); std::fill(sfy.begin(), sfy.end(), 0.0f);
+    std::fill(bfx.begin(), bfx.end(), 0.0f); std::fill(bfy.begin(), bfy.end(), 0.0f);
+    std::fill(dfx.begin(), dfx.end(), 0.0); std::fill(dfy.begin(), dfy.end(), 0.0);
+    if (components & 1) springs<float>(n, e, net_ptr, net_pins, x, y, attract, sfx.data(), sfy.data());
+    if (components & 2) orc_bh_record_f32(n, x, y, repel, theta, min_dist, bfx.data(), bfy.data(), nullptr, nthreads);
+    if ((components & 4) && G > 0)
+      orc_density_f64(n, x, y, die_w, die_h, G, target, dens_weight, dfx.data(), dfy.data(),
+                      nullptr, nullptr, nullptr, nullptr);
+    for (int32_t i = 0; i < n; ++i) {
+      const float gx = (sfx[i] + bfx[i]) + (float)dfx[i];
+      const float gy = (sfy[i] + bfy[i]) + (float)dfy[i];
+      if (fx) { fx[i] = gx; fy[i] = gy; }
+      if (iters > 0) {
+        vx[i] = (vx[i] + gx * dt) * damping;
+        vy[i] = (vy[i] + gy * dt) * damping;
+        x[i] = rust_clamp(x[i] + vx[i] * dt, 0.0f, 0.0f + die_w);
+        y[i] = rust_clamp(y[i] + vy[i] * dt, 0.0f, 0.0f + die_h);
+      }
+    }
+  }
+}
+
+// FP64 oracle of the flat step: springs, literal Barnes-Hut and density all
+// in double, integration in double.
+ORC_API void orc_flat_run_f64(int32_t n, int32_t e, const int32_t* net_ptr,
+                              const int32_t* net_pins, float die_w, float die_h,
+                              double attract, double repel, double damping, double dt,
+                              double theta, double min_dist, double dens_weight,
+                              double target, int32_t G, int32_t components, int32_t iters,
+                              double* x, double* y, double* vx, double* vy, double* fx,
+                              double* fy) {
+  std::vector<double> sfx(n), sfy(n), bfx(n), bfy(n), dfx(n), dfy(n), q(n, 1.0);
+  std::vector<float> xf(n), yf(n);
+  const int32_t steps = iters > 0 ? iters : 1;
+  for (int32_t it = 0; it < steps; ++it) {
+    std::fill(sfx.begin(), sfx.end(), 0.0); std::fill(sfy.begin(), sfy.end(), 0.0);
+    std::fill(bfx.begin(), bfx.end(), 0.0); std::fill(bfy.begin(), bfy.end(), 0.0);
+    std::fill(dfx.begin(), dfx.end(), 0.0); std::fill(dfy.begin(), dfy.end(), 0.0);
+    if (components & 1) springs<double>(n, e, net_ptr, net_pins, x, y, attract, sfx.data(), sfy.data());
+    if (components & 2) orc_bh_literal_f64(n, x, y, q.data(), repel, theta, min_dist, bfx.data(), bfy.data(), nullptr);
+    if ((components & 4) && G > 0) {
+      // The density bins of force_electrostatic.py:55-56 use the positions
+      // themselves; the f64 run keeps double positions, binned in double.
+      for (int32_t i = 0; i < n; ++i) { xf[i] = (float)x[i]; yf[i] = (float)y[i]; }
+      orc_density_f64(n, xf.data(), yf.data(), die_w, die_h, G, target, dens_weight,
+                      dfx.data(), dfy.data(), nullptr, nullptr, nullptr, nullptr);
+    }
+    for (int32_t i = 0; i < n; ++i) {
+      const double gx = (sfx[i] + bfx[i]) + dfx[i];
+      const double gy = (sfy[i] + bfy[i]) + dfy[i];
+      if (fx) { fx[i] = gx; fy[i] = gy; }
+      if (iters > 0) {
+        vx[i] = (vx[i] + gx * dt) * damping;
+        vy[i] = (vy[i] + gy * dt) * damping;
+        x[i] = rust_clamp(x[i] + vx[i] * dt, 0.0, (double)die_w);
+        y[i] = rust_clamp(y[i] + vy[i] * dt, 0.0, (double)die_h);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// compute_fiedler_vector (partitioner.rs:106-185): power iteration on the
+// Laplacian of the subgraph induced by `gates`.
+//   mode 0: f32 as written (f32 mean and norm),
+//   mode 1: f32 SpMV with FP64 mean and norm (the GPU's normalisation),
+//   and orc_power_iter_f64: everything in double.
+// v0: the initial vector (n values, the caller computes sin(i*2.71828) in
+// f32 as partitioner.rs:145-147 does); it is normalised first (149-155).
+ORC_API void orc_power_iter_f32(int32_t n, const int32_t* gates, int32_t num_gates,
+                                const int32_t* adj_ptr, const int32_t* adj,
+                                const float* v0, int32_t iters, int32_t mode, float* out) {
+  std::vector<int32_t> g2l(num_gates, -1);
+  for (int32_t i = 0; i < n; ++i) g2l[gates[i]] = i;
+  std::vector<float> degree(n, 0.0f);
+  std::vector<std::pair<int32_t, int32_t>> pairs;
+  for (int32_t li = 0; li < n; ++li) {
+    const int32_t g = gates[li];
+    for (int32_t k = adj_ptr[g]; k < adj_ptr[g + 1]; ++k) {
+      const int32_t lj = g2l[adj[k]];
+      if (lj < 0) continue;
+      pairs.push_back({li, lj});
+      degree[li] += 1.0f;
+    }
+  }
+  std::vector<float> v(v0, v0 + n), lv(n);
+  auto normalise = [&](std::vector<float>& a) {
+    if (mode == 0) {
+      float s = 0.0f;
+      for (float t : a) s += t * t;
+      const float nrm = std::sqrt(s);
+      if (nrm > 1e-10f)
+        for (float& t : a) t /= nrm;
+    } else {
+      double s = 0.0;
+      for (float t : a) s += (double)t * (double)t;
+      const double nrm = std::sqrt(s);
+      if (nrm > 1e-10)
+        for (float& t : a) t = (float)((double)t / nrm);
+    }
+  };
+  normalise(v);
+  for (int32_t it = 0; it < iters; ++it) {
+    for (int32_t i = 0; i < n; ++i) lv[i] = degree[i] * v[i];
+    for (auto& pr : pairs) lv[pr.first] -= v[pr.second];
+    if (mode == 0) {
+      float s = 0.0f;
+      for (float t : lv) s += t;
+      const float mean = s / (float)n;
+      for (float& t : lv) t -= mean;
+    } else {
+      double s = 0.0;
+      for (float t : lv) s += (double)t;
+      const float mean = (float)(s / (double)n);
+      for (float& t : lv) t -= mean;
+    }
+    normalise(lv);
+    v.swap(lv);
+  }
+  std::memcpy(out, v.data(), sizeof(float) * n);
+}
+
+ORC_API void orc_power_iter_f64(int32_t n, const int32_t* gates, int32_t num_gates,
+                                const int32_t* adj_ptr, const int32_t* adj,
+                                const double* v0, int32_t iters, double* out) {
+  std::vector<int32_t> g2l(num_gates, -1);
+  for (int32_t i = 0; i < n; ++i) g2l[gates[i]] = i;
+  std::vector<double> degree(n, 0.0);
+  std::vector<std::pair<int32_t, int32_t>> pairs;
+  for (int32_t li = 0; li < n; ++li) {
+    const int32_t g = gates[li];
+    for (int32_t k = adj_ptr[g]; k < adj_ptr[g + 1]; ++k) {
+      const int32_t lj = g2l[adj[k]];
+      if (lj < 0) continue;
+      pairs.push_back({li, lj});
+      degree[li] += 1.0;
+    }
+  }
+  std::vector<double> v(v0, v0 + n), lv(n);
+  auto normalise = [&](std::vector<double>& a) {
+    double s = 0.0;
+    for (double t : a) s += t * t;
+    const double nrm = std::sqrt(s);
+    if (nrm > 1e-10)
+      for (double& t : a) t /= nrm;
+  };
+  normalise(v);
+  for (int32_t it = 0; it < iters; ++it) {
+    for (int32_t i = 0; i < n; ++i) lv[i] = degree[i] * v[i];
+    for (auto& pr : pairs) lv[pr.first] -= v[pr.second];
+    double s = 0.0;
+    for (double t : lv) s += t;
+    const double mean = s / (double)n;
+    for (double& t : lv) t -= mean;
+    normalise(lv);
+    v.swap(lv);
+  }
+  std::memcpy(out, v.data(), sizeof(double) * n);
+}

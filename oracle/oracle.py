@@ -1,0 +1,237 @@
+"""ctypes loader of the CPU oracle (oracle/liboracle.so) and of oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker, never as the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_P = C.c_void_p
+_I32, _I64, _F, _D = C.c_int32, C.c_int64, C.c_float, C.c_double
+
+_SIGS = {
+    "orc_build_adjacency": (_I64, [_I32, _I32, _P, _P, _P, _P, _I64]),
+    "orc_place_leaves_f32": (None, [_I32, _P, _P, _P, _I32, _P, _P, _F, _F, _F, _F, _I32, _I32,
+                                    _P, _P, _P, _P, _P, _P, _I32]),
+    "orc_place_leaves_f64": (None, [_I32, _P, _P, _P, _I32, _P, _P, _F, _F, _F, _F, _I32, _I32,
+                                    _P, _P, _P, _P, _P, _P, _I32]),
+    "orc_congestion": (_I32, [_I32, _P, _P, _P, _P, _F, _F, _I32, _I32, _P]),
+    "orc_overflow_f32": (None, [_P, _I32, _I32, _F, _F, _P, _P]),
+    "orc_overflow_f64": (None, [_P, _I32, _I32, _F, _F, _P, _P]),
+    "orc_hpwl_f32": (_F, [_I32, _P, _P, _P, _P]),
+    "orc_hpwl_f64": (_D, [_I32, _P, _P, _P, _P]),
+    "orc_springs_f32": (None, [_I32, _I32, _P, _P, _P, _P, _F, _P, _P]),
+    "orc_springs_f64": (None, [_I32, _I32, _P, _P, _P, _P, _D, _P, _P]),
+    "orc_bh_geometry": (None, [_I32, _P, _P, _P, _P, _P]),
+    "orc_bh_record_f32": (None, [_I32, _P, _P, _F, _F, _F, _P, _P, _P, _I32]),
+    "orc_bh_literal_f64": (None, [_I32, _P, _P, _P, _D, _D, _D, _P, _P, _P]),
+    "orc_density_f64": (None, [_I32, _P, _P, _F, _F, _I32, _D, _D, _P, _P, _P, _P, _P, _P]),
+    "orc_flat_run_f32": (None, [_I32, _I32, _P, _P, _F, _F, _F, _F, _F, _F, _F, _F, _F, _F,
+                                _I32, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32]),
+    "orc_flat_run_f64": (None, [_I32, _I32, _P, _P, _F, _F, _D, _D, _D, _D, _D, _D, _D, _D,
+                                _I32, _I32, _I32, _P, _P, _P, _P, _P, _P]),
+    "orc_power_iter_f32": (None, [_I32, _P, _I32, _P, _P, _P, _I32, _I32, _P]),
+    "orc_power_iter_f64": (None, [_I32, _P, _I32, _P, _P, _P, _I32, _P]),
+}
+
+
+def _load(path, sigs):
+    lib = C.CDLL(path)
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load(os.path.join(HERE, "liboracle.so"), _SIGS)
+
+_REF_PATH = os.path.join(HERE, "_ref", "libbh_ref.so")
+ref = None
+if os.path.exists(_REF_PATH):
+    ref = _load(_REF_PATH, {"ref_bh_forces": (None, [_I32, _P, _P, _P, _D, _D, _D, _P, _P])})
+
+
+def p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def build_adjacency(n, net_ptr, net_pins):
+    net_ptr, net_pins = i32(net_ptr), i32(net_pins)
+    e = net_ptr.shape[0] - 1
+    cnt = lib.orc_build_adjacency(n, e, p(net_ptr), p(net_pins), None, None, 0)
+    aptr = np.zeros(n + 1, np.int32)
+    adj = np.zeros(max(1, cnt), np.int32)
+    lib.orc_build_adjacency(n, e, p(net_ptr), p(net_pins), p(aptr), p(adj), adj.shape[0])
+    return aptr, adj[:cnt]
+
+
+def place_leaves(leaf_ptr, leaf_gates, region, n, aptr, adj, attract=1.0, repel=0.5, damping=0.9,
+                 dt=0.1, iters=50, init=True, x=None, y=None, vx=None, vy=None, dtype=np.float32,
+                 nthreads=1):
+    """Returns (x, y, vx, vy, fx, fy) of the literal placer.rs leaf loop."""
+    x = np.zeros(n, dtype) if x is None else np.array(x, dtype, copy=True)
+    y = np.zeros(n, dtype) if y is None else np.array(y, dtype, copy=True)
+    vx = np.zeros(n, dtype) if vx is None else np.array(vx, dtype, copy=True)
+    vy = np.zeros(n, dtype) if vy is None else np.array(vy, dtype, copy=True)
+    fx = np.zeros(n, dtype)
+    fy = np.zeros(n, dtype)
+    fn = lib.orc_place_leaves_f32 if dtype == np.float32 else lib.orc_place_leaves_f64
+    adj = i32(adj) if len(adj) else np.zeros(1, np.int32)
+    fn(len(leaf_ptr) - 1, p(i32(leaf_ptr)), p(i32(leaf_gates)), p(f32(region)), n, p(i32(aptr)),
+       p(adj), attract, repel, damping, dt, iters, 1 if init else 0, p(x), p(y), p(vx), p(vy),
+       p(fx), p(fy), nthreads)
+    return x, y, vx, vy, fx, fy
+
+
+def congestion(net_ptr, net_pins, x, y, chip_w, chip_h, cols, rows):
+    out = np.zeros((rows, cols), np.int32)
+    net_ptr, net_pins = i32(net_ptr), i32(net_pins)
+    mism = lib.orc_congestion(net_ptr.shape[0] - 1, p(net_ptr), p(net_pins), p(f32(x)), p(f32(y)),
+                              chip_w, chip_h, cols, rows, p(out))
+    assert mism == 0, "h and v grids differ"
+    return out
+
+
+def overflow(counts, chip_w, chip_h, dtype=np.float64):
+    rows, cols = counts.shape
+    c = i32(counts)
+    if dtype == np.float32:
+        t, m = C.c_float(), C.c_float()
+        lib.orc_overflow_f32(p(c), cols, rows, chip_w, chip_h, C.byref(t), C.byref(m))
+    else:
+        t, m = C.c_double(), C.c_double()
+        lib.orc_overflow_f64(p(c), cols, rows, chip_w, chip_h, C.byref(t), C.byref(m))
+    return t.value, m.value
+
+
+def hpwl(net_ptr, net_pins, x, y, dtype=np.float64):
+    net_ptr, net_pins = i32(net_ptr), i32(net_pins)
+    e = net_ptr.shape[0] - 1
+    if dtype == np.float32:
+        return lib.orc_hpwl_f32(e, p(net_ptr), p(net_pins), p(f32(x)), p(f32(y)))
+    return lib.orc_hpwl_f64(e, p(net_ptr), p(net_pins), p(f64(x)), p(f64(y)))
+
+
+def springs(n, net_ptr, net_pins, x, y, w=1.0, dtype=np.float32):
+    net_ptr, net_pins = i32(net_ptr), i32(net_pins)
+    fx = np.zeros(n, dtype)
+    fy = np.zeros(n, dtype)
+    if dtype == np.float32:
+        lib.orc_springs_f32(n, net_ptr.shape[0] - 1, p(net_ptr), p(net_pins), p(f32(x)), p(f32(y)),
+                            w, p(fx), p(fy))
+    else:
+        lib.orc_springs_f64(n, net_ptr.shape[0] - 1, p(net_ptr), p(net_pins), p(f64(x)), p(f64(y)),
+                            w, p(fx), p(fy))
+    return fx, fy
+
+
+def bh_geometry(x, y):
+    root = np.zeros(4, np.float64)
+    d, s = C.c_int32(), C.c_int32()
+    x, y = f32(x), f32(y)
+    lib.orc_bh_geometry(x.shape[0], p(x), p(y), p(root), C.byref(d), C.byref(s))
+    return root, d.value, s.value
+
+
+def bh_record(x, y, k=0.5, theta=0.5, min_dist=0.1, nthreads=1):
+    x, y = f32(x), f32(y)
+    n = x.shape[0]
+    fx = np.zeros(n, np.float32)
+    fy = np.zeros(n, np.float32)
+    st = np.zeros(4, np.int64)
+    lib.orc_bh_record_f32(n, p(x), p(y), k, theta, min_dist, p(fx), p(fy), p(st), nthreads)
+    return fx, fy, {"interactions": int(st[0]), "opened": int(st[1]), "nodes": int(st[2]),
+                    "merge_depth": int(st[3])}
+
+
+def bh_literal(x, y, k=0.5, theta=0.5, min_dist=0.1, q=None):
+    x, y = f64(x), f64(y)
+    n = x.shape[0]
+    q = np.ones(n) if q is None else f64(q)
+    fx = np.zeros(n)
+    fy = np.zeros(n)
+    st = np.zeros(4, np.int64)
+    lib.orc_bh_literal_f64(n, p(x), p(y), p(q), k, theta, min_dist, p(fx), p(fy), p(st))
+    return fx, fy, {"interactions": int(st[0]), "opened": int(st[1]), "nodes": int(st[2])}
+
+
+def bh_reference(x, y, k=0.5, theta=0.5, min_dist=0.1, q=None):
+    """The reference's own BarnesHutTree.cpp (oracle/_ref), if it was built."""
+    if ref is None:
+        raise RuntimeError("oracle/_ref/libbh_ref.so not built")
+    x, y = f64(x), f64(y)
+    n = x.shape[0]
+    q = np.ones(n) if q is None else f64(q)
+    fx = np.zeros(n)
+    fy = np.zeros(n)
+    ref.ref_bh_forces(n, p(x), p(y), p(q), theta, k, min_dist, p(fx), p(fy))
+    return fx, fy
+
+
+def density(x, y, die_w, die_h, G, target=0.8, weight=1.0, fields=False):
+    x, y = f32(x), f32(y)
+    n = x.shape[0]
+    fx = np.zeros(n)
+    fy = np.zeros(n)
+    arrs = [np.zeros((G, G)) for _ in range(4)] if fields else [None] * 4
+    lib.orc_density_f64(n, p(x), p(y), die_w, die_h, G, target, weight, p(fx), p(fy),
+                        *[p(a) for a in arrs])
+    return (fx, fy, *arrs) if fields else (fx, fy)
+
+
+def flat_run(n, net_ptr, net_pins, die_w, die_h, x, y, vx=None, vy=None, iters=1, attract=1.0,
+             repel=0.5, damping=0.9, dt=0.1, theta=0.5, min_dist=0.1, dens_weight=1.0,
+             target=0.8, G=0, components=7, dtype=np.float32, nthreads=1):
+    """Flat step of the path of record (f32) or the FP64 oracle. iters=0: forces only."""
+    net_ptr, net_pins = i32(net_ptr), i32(net_pins)
+    x = np.array(x, dtype, copy=True)
+    y = np.array(y, dtype, copy=True)
+    vx = np.zeros(n, dtype) if vx is None else np.array(vx, dtype, copy=True)
+    vy = np.zeros(n, dtype) if vy is None else np.array(vy, dtype, copy=True)
+    fx = np.zeros(n, dtype)
+    fy = np.zeros(n, dtype)
+    e = net_ptr.shape[0] - 1
+    if dtype == np.float32:
+        lib.orc_flat_run_f32(n, e, p(net_ptr), p(net_pins), die_w, die_h, attract, repel, damping,
+                             dt, theta, min_dist, dens_weight, target, G, components, iters, p(x),
+                             p(y), p(vx), p(vy), p(fx), p(fy), nthreads)
+    else:
+        lib.orc_flat_run_f64(n, e, p(net_ptr), p(net_pins), die_w, die_h, attract, repel, damping,
+                             dt, theta, min_dist, dens_weight, target, G, components, iters, p(x),
+                             p(y), p(vx), p(vy), p(fx), p(fy))
+    return x, y, vx, vy, fx, fy
+
+
+def power_iter(gates, n_total, aptr, adj, v0, iters, mode=0, dtype=np.float32):
+    gates = i32(gates)
+    n = gates.shape[0]
+    adj = i32(adj) if len(adj) else np.zeros(1, np.int32)
+    if dtype == np.float32:
+        out = np.zeros(n, np.float32)
+        lib.orc_power_iter_f32(n, p(gates), n_total, p(i32(aptr)), p(adj), p(f32(v0)), iters, mode,
+                               p(out))
+    else:
+        out = np.zeros(n, np.float64)
+        lib.orc_power_iter_f64(n, p(gates), n_total, p(i32(aptr)), p(adj), p(f64(v0)), iters, p(out))
+    return out

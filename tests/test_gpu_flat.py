@@ -1,0 +1,177 @@
+"""Flat placement job on the GPU: springs, congestion, HPWL, overflow, BH, DCT, full step."""
+import numpy as np
+import pytest
+
+import fluxion as fxn
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def nrm_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def make(N, E, seed, **cfg):
+    circ, x, y = fxn.netgen.flat(N, E, seed)
+    job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(**cfg))
+    job.set_positions(x, y)
+    return circ, x, y, job
+
+
+fxn_ctx = [None]
+
+
+@pytest.fixture(autouse=True)
+def _ctx(ctx):
+    fxn_ctx[0] = ctx
+
+
+def test_springs_bit_exact():
+    print("case: springs 10k", flush=True)
+    circ, x, y, job = make(10000, 11000, 11)
+    gx, gy = job.forces(1)
+    rx, ry = orc.springs(circ.num_gates, circ.net_ptr, circ.net_pins, x, y, 1.0)
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+
+
+@pytest.mark.parametrize("N,E,C", [(10000, 11000, 64), (200000, 220000, 256)])
+def test_congestion_bit_exact(N, E, C):
+    print(f"case: congestion N={N} C={C}", flush=True)
+    circ, x, y, job = make(N, E, 21 + N, density_grid=0)
+    g = job.congestion(C, C)
+    r = orc.congestion(circ.net_ptr, circ.net_pins, x, y, 1000.0, 1000.0, C, C)
+    assert np.array_equal(g, r)
+    t, m = job.overflow(C, C)
+    rt, rm = orc.overflow(r, 1000.0, 1000.0)
+    assert t == pytest.approx(rt, rel=1e-12) and m == rm
+
+
+def test_congestion_kat_and_edges(ctx):
+    circ = fxn.Circuit.from_nets(4, [[0, 1, 2, 3]], 100.0, 100.0)
+    pl = fxn.Placement(np.array([10, 90, 10, 90], np.float32), np.array([10, 10, 90, 90], np.float32))
+    g = fxn.dispatch_congestion(ctx, 10, 10, pl, circ)
+    exp = np.zeros((10, 10), np.int32)
+    exp[1:, 1:] = 1
+    assert np.array_equal(g, exp)
+    # edge points, negatives, a 1-pin net, a non-square grid
+    circ = fxn.Circuit.from_nets(5, [[0, 1], [2], [3, 4], [1, 3]], 100.0, 100.0)
+    pl = fxn.Placement(np.array([-5, 150, 50, 100, 0], np.float32), np.array([100, 100, 50, 0, 100], np.float32))
+    for cols, rows in ((10, 10), (7, 13), (1, 1)):
+        g = fxn.dispatch_congestion(ctx, cols, rows, pl, circ)
+        r = orc.congestion(circ.net_ptr, circ.net_pins, pl.gate_x, pl.gate_y, 100.0, 100.0, cols, rows)
+        assert np.array_equal(g, r)
+
+
+def test_hpwl_kats_and_parity(ctx):
+    circ = fxn.Circuit.from_nets(4, [[0, 1, 2, 3]], 100.0, 100.0)
+    job = fxn.FlatJob(ctx, circ, fxn.PlacerConfig(density_grid=0))
+    job.set_positions(np.array([0, 10, 10, 0], np.float32), np.array([0, 0, 10, 10], np.float32))
+    assert job.hpwl() == 20.0
+    circ, x, y, job = make(200000, 220000, 31)
+    h = job.hpwl()
+    r = orc.hpwl(circ.net_ptr, circ.net_pins, x, y)
+    assert h == pytest.approx(r, rel=1e-12)
+
+
+@pytest.mark.parametrize("N", [1, 2, 1000, 20000, 200000])
+def test_barnes_hut_vs_record(N):
+    print(f"case: BH N={N}", flush=True)
+    circ, x, y, job = make(max(N, 2), max(N, 2), 41 + N, density_grid=0)
+    if N == 1:
+        circ = fxn.Circuit.from_nets(1, [[0]])
+        job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig())
+        x, y = np.array([3.0], np.float32), np.array([4.0], np.float32)
+        job.set_positions(x, y)
+    gx, gy = job.forces(2)
+    rx, ry, st = orc.bh_record(x, y, nthreads=8)
+    if N == 1:
+        assert gx[0] == 0 and gy[0] == 0
+        return
+    err = max(nrm_err(gx, rx), nrm_err(gy, ry))
+    same = np.mean((gx == rx) & (gy == ry))
+    print(f"BH N={N}: err {err:.2e}, bit-identical fraction {same:.4f}", flush=True)
+    assert err < 1e-4
+    s = job.bh_stats()
+    assert s["nodes"] == st["nodes"] and s["merge_depth"] == st["merge_depth"]
+    assert s["interactions"] == st["interactions"] and s["opened"] == st["opened"]
+
+
+def test_barnes_hut_colocated_and_clamped():
+    # many gates on the same point (die corner after clamping) and on the die edge
+    print("case: BH co-located", flush=True)
+    rng = np.random.default_rng(9)
+    N = 5000
+    x = rng.uniform(0, 1000, N).astype(np.float32)
+    y = rng.uniform(0, 1000, N).astype(np.float32)
+    x[:300] = 0.0
+    y[:300] = 0.0
+    x[300:600] = 1000.0
+    circ = fxn.Circuit.from_nets(N, [[i, (i + 1) % N] for i in range(N)])
+    job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig())
+    job.set_positions(x, y)
+    gx, gy = job.forces(2)
+    rx, ry, st = orc.bh_record(x, y)
+    assert max(nrm_err(gx, rx), nrm_err(gy, ry)) < 1e-4
+    lx, ly, _ = orc.bh_literal(x, y)
+    print(f"record vs literal f64: {max(nrm_err(rx, lx), nrm_err(ry, ly)):.2e}", flush=True)
+
+
+@pytest.mark.parametrize("G", [32, 64, 128, 512])
+def test_density_fields_vs_fp64(G):
+    print(f"case: DCT G={G}", flush=True)
+    N = 20000 if G < 512 else 200000
+    circ, x, y, job = make(N, N, 51 + G, density_grid=G)
+    rho, phi, ex, ey = job.density_fields()
+    rfx, rfy, rrho, rphi, rex, rey = orc.density(x, y, 1000.0, 1000.0, G, fields=True)
+    e = {k: nrm_err(a, b) for k, a, b in (("rho", rho, rrho), ("phi", phi, rphi), ("ex", ex, rex), ("ey", ey, rey))}
+    gx, gy = job.forces(4)
+    e["force"] = max(nrm_err(gx, rfx), nrm_err(gy, rfy))
+    print(f"DCT G={G}: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()), flush=True)
+    for k, v in e.items():
+        assert v < 1e-4, (k, v)
+
+
+def test_full_step_teacher_forced():
+    print("case: flat step teacher forced", flush=True)
+    N, G = 20000, 128
+    circ, x, y, job = make(N, 22000, 61, density_grid=G)
+    ox, oy, ovx, ovy = x, y, np.zeros(N, np.float32), np.zeros(N, np.float32)
+    for t in range(4):
+        job.set_positions(ox, oy)
+        job.set_velocities(ovx, ovy)
+        gx, gy = job.forces(7)
+        *_, rfx, rfy = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, ox, oy, ovx, ovy,
+                                    iters=0, G=G, nthreads=8)
+        err = max(nrm_err(gx, rfx), nrm_err(gy, rfy))
+        print(f"step {t}: force err {err:.2e}", flush=True)
+        assert err < 1e-4
+        ox, oy, ovx, ovy, *_ = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, ox, oy,
+                                            ovx, ovy, iters=25, G=G, nthreads=8)
+
+
+def test_run_hpwl_vs_fp64_and_graph_identical():
+    print("case: flat run vs fp64", flush=True)
+    N, G, T = 4000, 64, 100
+    circ, x, y, job = make(N, 4400, 71, density_grid=G)
+    job.run(T)
+    gx, gy = job.positions()
+    h = job.hpwl()
+    ox, oy, *_ = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, x, y, iters=T, G=G,
+                              dtype=np.float64)
+    h64 = orc.hpwl(circ.net_ptr, circ.net_pins, ox, oy)
+    print(f"HPWL gpu {h:.2f} fp64 {h64:.2f} rel {abs(h - h64) / h64:.2e}", flush=True)
+    assert abs(h - h64) / h64 < 5e-3
+    c_g = job.congestion(64, 64)
+    ot, om = job.overflow(64, 64)
+    c64 = orc.congestion(circ.net_ptr, circ.net_pins, ox.astype(np.float32), oy.astype(np.float32), 1000., 1000., 64, 64)
+    rt, rm = orc.overflow(c64, 1000.0, 1000.0)
+    print(f"overflow gpu {ot:.1f} fp64-run {rt:.1f}", flush=True)
+    # graph replay == eager, bit for bit
+    job2 = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(density_grid=G))
+    job2.set_positions(x, y)
+    job2.use_graph(True)
+    for _ in range(4):
+        job2.run(T // 4)
+    hx, hy = job2.positions()
+    assert np.array_equal(gx, hx) and np.array_equal(gy, hy)

@@ -1,0 +1,115 @@
+"""Batched leaf placement (placer.rs:66-69, 210-287) on the GPU vs the FP32 oracle."""
+import numpy as np
+import pytest
+
+import fluxion as fxn
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(L, seed, **kw):
+    circ, lp, gates, reg = fxn.netgen.leaves(L, seed, **kw)
+    aptr, adj = circ.build_adjacency()
+    return circ, lp, gates, reg, aptr, adj
+
+
+@pytest.mark.parametrize("L,iters", [(1, 5), (64, 50), (300, 20)])
+def test_exact_mode_bit_identical(ctx, L, iters):
+    print(f"case: exact L={L} iters={iters}", flush=True)
+    circ, lp, gates, reg, aptr, adj = _batch(L, 1000 + L)
+    pl = fxn.Placement.new(circ.num_gates)
+    fxn.place_leaves(ctx, lp, gates, reg, circ, pl, fxn.PlacerConfig(force_iterations=iters))
+    ox, oy, *_ = orc.place_leaves(lp, gates, reg, circ.num_gates, aptr, adj, iters=iters)
+    assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
+
+
+def test_exact_large_leaves_and_edge_sizes(ctx):
+    # leaves above 256 gates (several gates per thread), a 1-gate leaf, mixed packing
+    print("case: exact big/small leaves", flush=True)
+    sizes = [1, 2, 63, 64, 65, 257, 600, 1024, 3, 130]
+    nets, lp, start = [], [0], 0
+    rng = np.random.default_rng(7)
+    for n in sizes:
+        for _ in range(n):
+            if n >= 2:
+                nets.append(list(start + rng.choice(n, size=min(n, 3), replace=False)))
+        start += n
+        lp.append(start)
+    circ = fxn.Circuit.from_nets(start, nets)
+    reg = np.array([[50 + 100 * i, 50, 100, 100] for i in range(len(sizes))], np.float32)
+    gates = np.arange(start, dtype=np.int32)
+    pl = fxn.Placement.new(start)
+    fxn.place_leaves(ctx, lp, gates, reg, circ, pl, fxn.PlacerConfig(force_iterations=10))
+    aptr, adj = circ.build_adjacency()
+    ox, oy, *_ = orc.place_leaves(np.array(lp), gates, reg, start, aptr, adj, iters=10)
+    assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
+
+
+def test_permuted_gate_order(ctx):
+    # leaf gate lists in arbitrary global order (global_to_local of placer.rs:231-235)
+    print("case: permuted gates", flush=True)
+    circ, lp, gates, reg, aptr, adj = _batch(16, 77)
+    perm = np.random.default_rng(1).permutation(circ.num_gates).astype(np.int32)
+    pl = fxn.Placement.new(circ.num_gates)
+    fxn.place_leaves(ctx, lp, perm, reg, circ, pl, fxn.PlacerConfig(force_iterations=15))
+    ox, oy, *_ = orc.place_leaves(lp, perm, reg, circ.num_gates, aptr, adj, iters=15)
+    assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
+
+
+def test_fast_mode_teacher_forced_forces(ctx):
+    # per-iteration forces of the fast mode within 1e-4 (normwise) of the exact FP32 path,
+    # teacher-forced at iterations 0, 1, 10, 100, 299 of a 300-iteration run
+    print("case: fast teacher forced", flush=True)
+    circ, lp, gates, reg, aptr, adj = _batch(256, 55)
+    cfg = fxn.PlacerConfig(force_iterations=300)
+    n = circ.num_gates
+    x, y, vx, vy, *_ = orc.place_leaves(lp, gates, reg, n, aptr, adj, iters=0)
+    done = 0
+    for t in (0, 1, 10, 100, 299):
+        if t > done:
+            x, y, vx, vy, *_ = orc.place_leaves(lp, gates, reg, n, aptr, adj, iters=t - done,
+                                                init=False, x=x, y=y, vx=vx, vy=vy)
+            done = t
+        *_, rfx, rfy = orc.place_leaves(lp, gates, reg, n, aptr, adj, iters=1, init=False,
+                                        x=x, y=y, vx=vx, vy=vy)
+        *_, gfx, gfy = fxn.place_leaves_state(ctx, lp, gates, reg, circ, cfg, x, y, vx, vy,
+                                              iterations=1, fast=True, want_forces=True)
+        scale = max(np.abs(rfx).max(), np.abs(rfy).max())
+        err = max(np.abs(gfx - rfx).max(), np.abs(gfy - rfy).max()) / scale
+        print(f"iter {t}: fast-mode force error {err:.2e}", flush=True)
+        assert err < 1e-4
+        # exact mode: identical forces
+        *_, efx, efy = fxn.place_leaves_state(ctx, lp, gates, reg, circ, cfg, x, y, vx, vy,
+                                              iterations=1, fast=False, want_forces=True)
+        assert np.array_equal(efx, rfx) and np.array_equal(efy, rfy)
+
+
+def test_full_batch_4096_exact(ctx):
+    # The 4,096-leaf workload (16-256 gates, degree 2-8, 100x100 regions), 300 iterations.
+    print("case: 4096 leaves x 300 exact", flush=True)
+    circ, lp, gates, reg, aptr, adj = _batch(4096, fxn.netgen.SEED_BASE + 3)
+    pl = fxn.Placement.new(circ.num_gates)
+    ms = fxn.place_leaves(ctx, lp, gates, reg, circ, pl, fxn.PlacerConfig(force_iterations=300))
+    print(f"kernel {ms:.2f} ms", flush=True)
+    ox, oy, *_ = orc.place_leaves(lp, gates, reg, circ.num_gates, aptr, adj, iters=300, nthreads=16)
+    assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
+    ox64, oy64, *_ = orc.place_leaves(lp, gates, reg, circ.num_gates, aptr, adj, iters=300,
+                                      dtype=np.float64, nthreads=16)
+    h32 = orc.hpwl(circ.net_ptr, circ.net_pins, pl.gate_x, pl.gate_y)
+    h64 = orc.hpwl(circ.net_ptr, circ.net_pins, ox64, oy64)
+    print(f"HPWL gpu {h32:.3f} fp64 oracle {h64:.3f} rel {abs(h32 - h64) / h64:.2e}", flush=True)
+    assert abs(h32 - h64) / h64 < 5e-3
+
+
+def test_invalid_inputs_rejected(ctx):
+    circ, lp, gates, reg, aptr, adj = _batch(2, 5)
+    bad = gates.copy()
+    bad[0] = circ.num_gates + 5
+    pl = fxn.Placement.new(circ.num_gates)
+    with pytest.raises(fxn.FluxionError):
+        fxn.place_leaves(ctx, lp, bad, reg, circ, pl, fxn.PlacerConfig())
+    dup = gates.copy()
+    dup[1] = dup[0]
+    with pytest.raises(fxn.FluxionError):
+        fxn.place_leaves(ctx, lp, dup, reg, circ, pl, fxn.PlacerConfig())

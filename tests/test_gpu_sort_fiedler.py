@@ -1,0 +1,77 @@
+"""GPU radix sort (bit-exact stable permutations) and power iteration."""
+import numpy as np
+import pytest
+
+import fluxion as fxn
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 7, 2048, 2049, 100000, 1000003])
+def test_sort_u64_stable(ctx, n):
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True)
+    k[: n // 3] = k[n // 2] if n > 2 else k[: n // 3]  # ties
+    k[-(n // 5 or 1):] &= np.uint64(0xFFFF)
+    perm = fxn.sort_u64(ctx, k)
+    assert np.array_equal(perm, np.argsort(k, kind="stable").astype(np.uint32))
+
+
+def test_sort_f32_partial_cmp(ctx):
+    rng = np.random.default_rng(1)
+    k = rng.standard_normal(50000).astype(np.float32)
+    k[::7] = 0.0
+    k[::11] = -0.0  # -0.0 == +0.0: ties keep input order
+    k[::13] = np.float32(1.5)
+    k[5] = np.inf
+    k[6] = -np.inf
+    perm = fxn.sort_f32(ctx, k)
+    canon = np.where(k == 0, np.float32(0), k)
+    assert np.array_equal(perm, np.argsort(canon, kind="stable").astype(np.uint32))
+    bad = k.copy()
+    bad[3] = np.nan
+    with pytest.raises(fxn.FluxionError):
+        fxn.sort_f32(ctx, bad)
+
+
+def test_split_by_fiedler(ctx):
+    rng = np.random.default_rng(2)
+    n = 1001
+    f = rng.standard_normal(n).astype(np.float32)
+    g = rng.permutation(5000)[:n].astype(np.int32)
+    left, right = fxn.split_by_fiedler(ctx, f, g, 0.4)
+    order = g[np.argsort(f, kind="stable")]
+    min_size = int(np.ceil(np.float32(n) * np.float32(0.4)))
+    split = min(max(n // 2, min_size), n - min_size)
+    assert np.array_equal(left, order[:split]) and np.array_equal(right, order[split:])
+
+
+@pytest.mark.parametrize("n,iters", [(2, 30), (2000, 30), (200000, 200)])
+def test_power_iteration(ctx, n, iters):
+    print(f"case: power iteration n={n}", flush=True)
+    circ, _, _ = fxn.netgen.flat(max(n, 2), max(n, 2) * 11 // 10, 500 + n)
+    gates = np.arange(n, dtype=np.int32)
+    aptr, adj = circ.build_adjacency()
+    v0 = fxn.fiedler_init(n)
+    v, rq = fxn.compute_fiedler_vector(ctx, circ, gates, iters, rayleigh=True)
+    r1 = orc.power_iter(gates, circ.num_gates, aptr, adj, v0, iters, mode=1)
+    r0 = orc.power_iter(gates, circ.num_gates, aptr, adj, v0, iters, mode=0)
+    r64 = orc.power_iter(gates, circ.num_gates, aptr, adj, v0, iters, dtype=np.float64)
+    e1 = np.abs(v - r1).max() / np.abs(r1).max()
+    e0 = np.abs(v - r0).max() / np.abs(r0).max()
+    e64 = np.abs(v - r64).max() / np.abs(r64).max()
+    print(f"n={n}: vs fp64-normalised f32 {e1:.2e}, vs literal f32 {e0:.2e}, vs fp64 {e64:.2e}, "
+          f"rayleigh {rq:.6f}", flush=True)
+    assert e1 < 1e-5
+    assert e64 < 1e-3
+
+
+def test_power_iteration_subgraph_order(ctx):
+    # a subset of gates in shuffled order (partitioner.rs:116-137 local ids)
+    circ, _, _ = fxn.netgen.flat(5000, 5500, 77)
+    gates = np.random.default_rng(3).permutation(5000)[:1500].astype(np.int32)
+    aptr, adj = circ.build_adjacency()
+    v = fxn.compute_fiedler_vector(ctx, circ, gates, 30)
+    r = orc.power_iter(gates, circ.num_gates, aptr, adj, fxn.fiedler_init(1500), 30, mode=1)
+    assert np.abs(v - r).max() < 1e-5

@@ -1,0 +1,74 @@
+"""C-ABI library: loads, exports every declared symbol; host utilities (no GPU compute)."""
+import os
+import re
+
+import numpy as np
+
+import fluxion as fxn
+from fluxion import _lib
+from oracle import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_header_symbols_exported():
+    hdr = open(os.path.join(ROOT, "include", "fluxion_hip.h")).read()
+    declared = set(re.findall(r"\b(fx_[a-z0-9_]+)\(", hdr))
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(_lib.lib, name), name
+    assert declared == set(_lib.SYMBOLS)
+
+
+def test_build_adjacency_matches_oracle():
+    circ, _, _ = fxn.netgen.flat(3000, 3300, 123)
+    a1, j1 = circ.build_adjacency()
+    a2, j2 = orc.build_adjacency(circ.num_gates, circ.net_ptr, circ.net_pins)
+    assert np.array_equal(a1, a2) and np.array_equal(j1, j2)
+
+
+def test_netgen_deterministic_and_distribution():
+    c1, x1, y1 = fxn.netgen.flat(20000, 22000, fxn.netgen.SEED_BASE + 1)
+    c2, x2, y2 = fxn.netgen.flat(20000, 22000, fxn.netgen.SEED_BASE + 1)
+    assert np.array_equal(c1.net_pins, c2.net_pins) and np.array_equal(x1, x2)
+    deg = np.diff(c1.net_ptr)
+    assert abs((deg == 2).mean() - 0.6) < 0.02
+    assert abs(((deg >= 3) & (deg <= 5)).mean() - 0.3) < 0.02
+    assert abs((deg >= 6).mean() - 0.1) < 0.01 and deg.max() <= 20
+    assert x1.min() >= 0 and x1.max() < 1000
+    # no duplicate pins inside a net
+    for e in range(0, 22000, 97):
+        p = c1.net_pins[c1.net_ptr[e]:c1.net_ptr[e + 1]]
+        assert len(set(p.tolist())) == len(p)
+    # locality: most pins of a net share one 64-gate block
+    blk = c1.net_pins // 64
+    same = []
+    for e in range(0, 22000, 13):
+        b = blk[c1.net_ptr[e]:c1.net_ptr[e + 1]]
+        same.append(np.bincount(b).max() / len(b))
+    assert np.mean(same) > 0.8
+
+
+def test_netgen_leaves_shape():
+    circ, lp, gates, reg = fxn.netgen.leaves(4096, fxn.netgen.SEED_BASE + 3)
+    n = np.diff(lp)
+    assert n.min() >= 16 and n.max() <= 256 and len(n) == 4096
+    assert abs(np.log(n).mean() - (np.log(16) + np.log(256)) / 2) < 0.05
+    assert np.all(reg[:, 2] == 100.0)
+    # nets stay inside their leaf
+    leaf_of = np.repeat(np.arange(4096), n)
+    for e in range(0, circ.num_nets, 101):
+        p = circ.net_pins[circ.net_ptr[e]:circ.net_ptr[e + 1]]
+        assert len(set(leaf_of[p].tolist())) == 1
+
+
+def test_no_gpu_context_is_an_error_not_a_fallback():
+    import torch
+    if torch.cuda.is_available():
+        return
+    try:
+        fxn.GpuContext.new(0)
+    except fxn.FluxionError as e:
+        assert e.code in (_lib.FX_E_DEVICE, _lib.FX_E_INVALID)
+    else:
+        raise AssertionError("context created without a GPU")

@@ -1,0 +1,253 @@
+"""Known-answer and closed-form checks that pin the CPU oracle (no GPU)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+
+def nets_csr(nets):
+    ptr = np.zeros(len(nets) + 1, np.int32)
+    ptr[1:] = np.cumsum([len(n) for n in nets])
+    return ptr, np.array([g for n in nets for g in n], np.int32)
+
+
+def test_hpwl_square_20():
+    # placement.rs:86-97: 4-pin net on a 10x10 square -> 20.0
+    ptr, pins = nets_csr([[0, 1, 2, 3]])
+    x = np.array([0, 10, 10, 0], np.float32)
+    y = np.array([0, 0, 10, 10], np.float32)
+    assert orc.hpwl(ptr, pins, x, y, np.float32) == pytest.approx(20.0, abs=0.01)
+    assert orc.hpwl(ptr, pins, x, y) == 20.0
+
+
+def test_hpwl_two_pin_30():
+    # test_benchmarks.py:249-265: (0,0)-(10,20) -> 30
+    ptr, pins = nets_csr([[0, 1]])
+    assert orc.hpwl(ptr, pins, [0, 10], [0, 20]) == 30.0
+
+
+def test_hpwl_skips_single_pin_nets():
+    ptr, pins = nets_csr([[0], [0, 1]])
+    assert orc.hpwl(ptr, pins, [5, 7], [1, 4]) == 5.0
+
+
+def test_adjacency_order():
+    # circuit.rs:200-216: nets [0,1], [1,2] -> adj[0] = [1], adj[1] = [0, 2]
+    ptr, pins = nets_csr([[0, 1], [1, 2]])
+    aptr, adj = orc.build_adjacency(3, ptr, pins)
+    assert adj[aptr[0]:aptr[1]].tolist() == [1]
+    assert adj[aptr[1]:aptr[2]].tolist() == [0, 2]
+    assert adj[aptr[2]:aptr[3]].tolist() == [1]
+
+
+def test_adjacency_dedup_first_insertion_order():
+    ptr, pins = nets_csr([[0, 2, 1], [1, 0], [3, 0]])
+    aptr, adj = orc.build_adjacency(4, ptr, pins)
+    assert adj[aptr[0]:aptr[1]].tolist() == [2, 1, 3]
+    assert adj[aptr[1]:aptr[2]].tolist() == [0, 2]
+
+
+def test_congestion_corner_kat():
+    # congestion.rs:100-117 strengthened: corners of 100x100, 10x10 cells.
+    ptr, pins = nets_csr([[0, 1, 2, 3]])
+    x = np.array([10, 90, 10, 90], np.float32)
+    y = np.array([10, 10, 90, 90], np.float32)
+    c = orc.congestion(ptr, pins, x, y, 100.0, 100.0, 10, 10)
+    exp = np.zeros((10, 10), np.int32)
+    exp[1:10, 1:10] = 1
+    assert np.array_equal(c, exp)
+    tot, mx = orc.overflow(c, 100.0, 100.0)
+    assert tot == 0.0 and mx == 0.0  # capacity (10+10)/2 = 10 (grid.rs:31)
+
+
+def test_cell_at_kats_and_edges():
+    # grid.rs:137-151: cell_at(5,5) = (0,0), cell_at(95,95) = (9,9); edges clamp
+    ptr, pins = nets_csr([[0, 1]])
+    c = orc.congestion(ptr, pins, [5, 5.5], [5, 5.5], 100, 100, 10, 10)
+    assert c[0, 0] == 1 and c.sum() == 1
+    c = orc.congestion(ptr, pins, [95, 95], [95, 96], 100, 100, 10, 10)
+    assert c[9, 9] == 1 and c.sum() == 1
+    # at/after the die edge -> last cell; negative -> cell 0
+    c = orc.congestion(ptr, pins, [-5, 150], [100, 100], 100, 100, 10, 10)
+    assert c[9, :].sum() == 10 and c.sum() == 10
+
+
+def test_overflow_formula():
+    c = np.array([[0, 6], [10, 3]], np.int32)  # cap = (50+50)/2 = 50 on a 100x100 die, 2x2
+    t, m = orc.overflow(c, 100.0, 100.0)
+    assert t == 0 and m == 0
+    t, m = orc.overflow(c * 10, 100.0, 100.0)  # 2*c - 50: 70, 150, 10 -> positives
+    assert t == 70 + 150 + 10 and m == 150
+
+
+def _leaf(n, nets, w=100.0, h=100.0, cx=50.0, cy=50.0):
+    ptr, pins = nets_csr(nets)
+    aptr, adj = orc.build_adjacency(n, ptr, pins)
+    return np.array([0, n], np.int32), np.arange(n, dtype=np.int32), np.array([[cx, cy, w, h]], np.float32), aptr, adj
+
+
+def test_leaf_init_positions():
+    # placer.rs:226-227 (after 0 iterations)
+    lp, g, reg, aptr, adj = _leaf(5, [])
+    x, y, *_ = orc.place_leaves(lp, g, reg, 5, aptr, adj, iters=0)
+    np.testing.assert_array_equal(x, np.float32(0) + (np.arange(5, dtype=np.float32) / np.float32(5)) * np.float32(100))
+    exp_y = (np.array([(i * 7) % 5 for i in range(5)], np.float32) / np.float32(5)) * np.float32(100)
+    np.testing.assert_array_equal(y, exp_y)
+
+
+def test_leaf_two_gate_closed_form():
+    # One iteration of two connected gates, computed by hand in float64.
+    lp, g, reg, aptr, adj = _leaf(2, [[0, 1]])
+    x, y, vx, vy, fx, fy = orc.place_leaves(lp, g, reg, 2, aptr, adj, iters=1, dtype=np.float64)
+    x0 = np.array([0.0, 50.0]); y0 = np.array([0.0, 50.0])
+    # PlacerConfig fields are f32 (placer.rs:13-19): the FP64 oracle uses their f32 values
+    a, r, dt, damp = (float(np.float32(v)) for v in (1.0, 0.5, 0.1, 0.9))
+    spring = a * np.array([x0[1] - x0[0], x0[0] - x0[1]])
+    dx, dy = x0[0] - x0[1], y0[0] - y0[1]
+    d = max(np.hypot(dx, dy), 0.1)
+    rep = r / d**2 * dx / d
+    exp_fx = spring + np.array([rep, -rep])
+    np.testing.assert_allclose(fx, exp_fx, rtol=1e-12)
+    exp_vx = (0 + exp_fx * dt) * damp
+    np.testing.assert_allclose(vx, exp_vx, rtol=1e-12)
+    np.testing.assert_allclose(x, np.clip(x0 + exp_vx * dt, 0, 100), rtol=1e-12)
+
+
+def test_leaf_repulsion_clamped_distance():
+    # coincident gates: d clamps to 0.1, force finite and zero along dx = 0
+    lp, g, reg, aptr, adj = _leaf(2, [], w=0.0, h=0.0)
+    x, y, vx, vy, fx, fy = orc.place_leaves(lp, g, reg, 2, aptr, adj, iters=1, dtype=np.float64)
+    assert np.all(np.isfinite(fx)) and np.all(fx == 0)
+
+
+def test_leaf_velocity_decay_recurrence():
+    # no forces (single gate): v_{t+1} = 0.9 v_t, x_{t+1} = clamp(x_t + 0.1 v_{t+1})
+    lp, g, reg, aptr, adj = _leaf(1, [], w=1000.0, h=1000.0, cx=500.0, cy=500.0)
+    x, y, vx, vy, *_ = orc.place_leaves(lp, g, reg, 1, aptr, adj, iters=10, init=False,
+                                        x=[100.0], y=[200.0], vx=[10.0], vy=[-5.0], dtype=np.float64)
+    damp, dt = float(np.float32(0.9)), float(np.float32(0.1))
+    v = 10.0 * damp ** np.arange(1, 11)
+    assert vx[0] == pytest.approx(v[-1], rel=1e-12)
+    assert x[0] == pytest.approx(100.0 + dt * v.sum(), rel=1e-12)
+
+
+def test_leaf_newton_third_law():
+    rng = np.random.default_rng(0)
+    n = 40
+    nets = [list(rng.choice(n, size=rng.integers(2, 6), replace=False)) for _ in range(50)]
+    lp, g, reg, aptr, adj = _leaf(n, nets)
+    *_, fx, fy = orc.place_leaves(lp, g, reg, n, aptr, adj, iters=1, dtype=np.float64)
+    assert abs(fx.sum()) < 1e-9 * np.abs(fx).max() and abs(fy.sum()) < 1e-9 * np.abs(fy).max()
+
+
+def _brute_repulsion(x, y, k, md):
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    d = np.maximum(np.hypot(dx, dy), md)
+    f = k / d**2
+    np.fill_diagonal(f, 0)
+    return (f * dx / d).sum(1), (f * dy / d).sum(1)
+
+
+def test_bh_theta_zero_equals_brute_force():
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0, 100, 300).astype(np.float32)
+    y = rng.uniform(0, 100, 300).astype(np.float32)
+    bx, by = _brute_repulsion(x.astype(np.float64), y.astype(np.float64), 0.5, 0.1)
+    lx, ly, _ = orc.bh_literal(x, y, theta=0.0)
+    np.testing.assert_allclose(lx, bx, rtol=1e-9, atol=1e-12)
+    rx, ry, _ = orc.bh_record(x, y, theta=0.0)
+    scale = np.abs(bx).max()
+    assert np.abs(rx - bx).max() / scale < 1e-5 and np.abs(ry - by).max() / scale < 1e-5
+
+
+def test_bh_literal_matches_reference_source():
+    if orc.ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(2)
+    x = rng.uniform(0, 1000, 3000)
+    y = rng.uniform(0, 1000, 3000)
+    x[:20] = 0.0  # co-located points exercise the merge rule
+    y[:20] = 0.0
+    rx, ry = orc.bh_reference(x, y)
+    lx, ly, _ = orc.bh_literal(x, y)
+    assert np.array_equal(rx, lx) and np.array_equal(ry, ly)
+
+
+def test_bh_record_vs_literal_and_accuracy_band():
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 1000, 5000).astype(np.float32)
+    y = rng.uniform(0, 1000, 5000).astype(np.float32)
+    rx, ry, st = orc.bh_record(x, y)
+    lx, ly, st2 = orc.bh_literal(x, y)
+    scale = max(np.abs(lx).max(), np.abs(ly).max())
+    err = max(np.abs(rx - lx).max(), np.abs(ry - ly).max()) / scale
+    assert err < 1e-4, err
+    assert st["nodes"] == st2["nodes"]
+    # theta = 0.5 vs exact O(N^2): the "~1% error" band of barnes_hut.py:16
+    bx, by = _brute_repulsion(x.astype(np.float64), y.astype(np.float64), 0.5, 0.1)
+    e = np.hypot(lx - bx, ly - by).sum() / np.hypot(bx, by).sum()
+    assert e < 0.05, e
+
+
+def test_bh_root_aggregates():
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 1000, 1000).astype(np.float32)
+    y = rng.uniform(0, 1000, 1000).astype(np.float32)
+    root, depth, sbits = orc.bh_geometry(x, y)
+    assert root[2] - root[0] == pytest.approx(root[3] - root[1])
+    assert root[0] <= x.min() - 1.0 + 1e-9 and root[2] >= x.max() + 1.0 - 1e-9
+    assert depth == 32 and 0 < sbits <= 40
+
+
+def test_power_iteration_path_graph_top_eigenvector():
+    # Plain power iteration on L (partitioner.rs:157-182) converges to L's top
+    # eigenvector; for a path graph it is cos(pi (n-1)(i+1/2)/n).
+    n = 8
+    gates = np.arange(n, dtype=np.int32)
+    ptr, pins = nets_csr([[i, i + 1] for i in range(n - 1)])
+    aptr, adj = orc.build_adjacency(n, ptr, pins)
+    v0 = np.sin(np.arange(n, dtype=np.float32) * np.float32(2.71828)).astype(np.float32)
+    v = orc.power_iter(gates, n, aptr, adj, v0, 400, dtype=np.float64)
+    top = np.cos(np.pi * (n - 1) * (np.arange(n) + 0.5) / n)
+    top /= np.linalg.norm(top)
+    assert abs(abs(v @ top) - 1.0) < 1e-8
+
+
+def test_power_iteration_matches_eigh():
+    rng = np.random.default_rng(5)
+    n = 30
+    nets = [list(rng.choice(n, size=rng.integers(2, 5), replace=False)) for _ in range(40)]
+    ptr, pins = nets_csr(nets)
+    aptr, adj = orc.build_adjacency(n, ptr, pins)
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, adj[aptr[i]:aptr[i + 1]]] = 1
+    L = np.diag(A.sum(1)) - A
+    w, V = np.linalg.eigh(L)
+    v0 = np.sin(np.arange(n, dtype=np.float32) * np.float32(2.71828)).astype(np.float32)
+    v = orc.power_iter(np.arange(n), n, aptr, adj, v0, 3000, dtype=np.float64)
+    assert abs(abs(v @ V[:, -1]) - 1.0) < 1e-6
+    v32 = orc.power_iter(np.arange(n), n, aptr, adj, v0, 30, mode=0)
+    v64 = orc.power_iter(np.arange(n), n, aptr, adj, v0, 30, dtype=np.float64)
+    assert np.abs(v32 - v64).max() < 1e-5
+
+
+def test_springs_centroid_equals_clique_pairs():
+    rng = np.random.default_rng(6)
+    n = 50
+    nets = [list(rng.choice(n, size=rng.integers(2, 9), replace=False)) for _ in range(60)]
+    ptr, pins = nets_csr(nets)
+    x = rng.uniform(0, 100, n)
+    y = rng.uniform(0, 100, n)
+    fx, fy = orc.springs(n, ptr, pins, x, y, 1.0, dtype=np.float64)
+    ex = np.zeros(n)
+    ey = np.zeros(n)
+    for net in nets:
+        for i in net:
+            for j in net:
+                if i != j:
+                    ex[i] += x[j] - x[i]
+                    ey[i] += y[j] - y[i]
+    np.testing.assert_allclose(fx, ex, rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(fy, ey, rtol=1e-10, atol=1e-9)
