@@ -61,7 +61,7 @@ def phase_roofline(phase, N, E, P, G, stats, avg_ms):
             "frac": ach / PEAK_HBM_GBS, "algorithmic": f"{bytes_per:.4g} B"}
 
 
-def cpu_baseline(circ, x, y, G, seconds):
+def cpu_baseline(circ, x, y, G, dw, seconds):
     """FP32 CPU path of record (the oracle) on a bounded sample of iterations."""
     from oracle import oracle as orc
     n = circ.num_gates
@@ -70,7 +70,8 @@ def cpu_baseline(circ, x, y, G, seconds):
     xx, yy, vx, vy = x, y, None, None
     while True:
         xx, yy, vx, vy, *_ = orc.flat_run(n, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, xx, yy,
-                                          vx, vy, iters=1, G=G, nthreads=1)
+                                          vx, vy, iters=1, G=G, dens_weight=dw,
+                                          nthreads=1)
         its += 1
         el = time.perf_counter() - t0
         if el >= seconds or its >= 50:
@@ -88,6 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="flat_200k", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--density-weight", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true",
                     help="replay a captured hipGraph per step (no per-kernel events)")
@@ -107,7 +109,7 @@ def main():
     circ, x0, y0 = fxn.netgen.flat(N, E, fxn.netgen.SEED_BASE + seed_off)
     P = int(circ.net_ptr[-1])
     ctx = fxn.GpuContext.new(0)
-    cfg = fxn.PlacerConfig(density_grid=G)
+    cfg = fxn.PlacerConfig(density_grid=G, density_weight=args.density_weight)
     job = fxn.FlatJob(ctx, circ, cfg)
     job.set_positions(x0, y0)
 
@@ -155,8 +157,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic seeded netlist (no dataset)",
         "config": {"workload": args.workload, "gates": N, "nets": E, "pins": P,
-                   "density_grid": G, "congestion_grid": C, "theta": 0.5,
-                   "dct": "split-bf16 3-pass MFMA, fp32 accumulate",
+                   "density_grid": G, "density_weight": args.density_weight,
+                   "congestion_grid": C, "theta": 0.5,
+                   "dct": "split-bf16 MFMA (3-part split, 6 passes), fp32 accumulate",
                    "hpwl": "fp64", "execution": "graph" if args.graph else "eager+events"},
         "final_hpwl": hpwl,
         "routing_overflow": {"total": tot_of, "max": max_of},
@@ -172,7 +175,7 @@ def main():
         rf["traffic"] = None
         out["roofline"] = rf
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(circ, x0, y0, G, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(circ, x0, y0, G, args.density_weight, args.cpu_seconds)
         out["cpu_baseline"]["ratio_gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     job.close()
     ctx.close()

@@ -50,7 +50,9 @@ typedef struct fx_params {
   float density_weight;
   float target_density;
   int32_t density_grid;
-  int32_t dct_passes;  /* split-BF16 MFMA passes: 3 (hi*hi, hi*lo, lo*hi) or 4 (+lo*lo) */
+  int32_t dct_passes;  /* forward-transform split-BF16 MFMA passes: 3 (hi*hi, hi*lo, lo*hi),
+                          4 (+lo*lo) or 6 (three-part split; default). The inverse
+                          transform always uses the six-pass three-part split. */
   int32_t flags;       /* FX_FLAG_* */
 } fx_params;
 

@@ -13,12 +13,25 @@
 // into the force-combine kernel.
 //
 // Each product is a GxGxG GEMM on v_mfma_f32_32x32x8_bf16 with FP32 operands
-// split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi)); the product is
-// hi*hi + hi*lo + lo*hi (+ lo*lo with 4 passes) accumulated in FP32, which
-// recovers close to FP32 accuracy. One 64-lane wave owns a 32x32 output
-// tile. The bin counts are deposited with int32 atomics (exact, order-free)
-// and converted to a = count*w - target while loading the first GEMM's
-// operand; the eigenvalue division is the second GEMM's epilogue.
+// split into bf16 parts, accumulated in FP32:
+//   forward (C a C^T): x = hi + lo, products hi*hi + hi*lo + lo*hi (3 passes;
+//     4 with lo*lo), or with dct_passes = 6 the three-part split below;
+//   inverse (C^T phi_hat C): x = hi + mid + lo, the six products whose
+//     orders sum to at most 2 (hi*hi, hi*mid, mid*hi, hi*lo, mid*mid, lo*hi).
+// Measured on MI355X at G=512 against the FP64 oracle: 3-pass forward gives
+// E within 1.02e-4 (normwise), 6-pass forward within 1e-5.
+// The inverse needs the extra part: the reference's eigenvalue grows as k^4,
+// so phi is dominated by the lowest modes and its centred differences are
+// ~G/pi times smaller than phi itself; FP32-level accuracy of phi is what
+// keeps E within 1e-4.
+// The forward input is taken about the mean bin density, a = (count - N/G^2)
+// * w: this differs from rho - target by a constant, which only changes the
+// DC coefficient, and the DC coefficient is discarded (phi_hat[0][0] = 0), so
+// phi is unchanged; removing the mean keeps the split precision relative to
+// the density fluctuations rather than to the background.
+// One 64-lane wave owns a 32x32 output tile; operands for the next K step
+// are loaded while the current step's MFMAs issue. Bin counts are deposited
+// with int32 atomics (exact, order-free).
 
 #include <cmath>
 #include <vector>
@@ -53,12 +66,17 @@ __host__ __device__ inline float bf16_to_f(uint16_t h) {
 #endif
 }
 
-__device__ __forceinline__ void split4(const float v[4], short4_t& hi, short4_t& lo) {
+template <int SPLIT>
+__device__ __forceinline__ void split4(const float v[4], short4_t& p0, short4_t& p1,
+                                       short4_t& p2) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const uint16_t h = bf16_rne(v[k]);
-    hi[k] = (short)h;
-    lo[k] = (short)bf16_rne(v[k] - bf16_to_f(h));
+    const float r = v[k] - bf16_to_f(h);
+    const uint16_t m = bf16_rne(r);
+    p0[k] = (short)h;
+    p1[k] = (short)m;
+    if (SPLIT == 3) p2[k] = (short)bf16_rne(r - bf16_to_f(m));
   }
 }
 
@@ -68,62 +86,80 @@ enum Epilogue { kStore = 0, kEigen = 1 };
 struct GemmArgs {
   int G;
   // left operand A[M][K]
-  const uint16_t* a_hi;  // kConst: bf16 row-major [row][k]
-  const uint16_t* a_lo;
-  const float* a_f;      // kData: fp32 row-major [row][k]
+  const uint16_t* a_p[3];  // kConst: bf16 parts, row-major [row][k]
+  const float* a_f;        // kData: fp32 row-major [row][k]
   // right operand B[K][N]
-  const uint16_t* bt_hi; // kConst: stored transposed, bt[col][k]
-  const uint16_t* bt_lo;
-  const float* b_f;      // kData: fp32 row-major [k][col]
-  const int* b_cnt;      // kCounts: int32 row-major [k][col]
-  float cnt_w, target;
-  const float* eig;      // kEigen: lambda [row][col]
+  const uint16_t* bt_p[3];  // kConst: bf16 parts stored transposed, bt[col][k]
+  const float* b_f;         // kData: fp32 row-major [k][col]
+  const int* b_cnt;         // kCounts: int32 row-major [k][col]
+  float cnt_w, mean;
+  const float* eig;  // kEigen: lambda [row][col]
   float* out;
-  int passes;
+  int passes;  // SPLIT == 2: 3 or 4
 };
 
-template <int AK, int BK, int EPI>
+template <int KIND, int SPLIT, bool LEFT>
+__device__ __forceinline__ void load_operand(const GemmArgs& g, int idx_row, int k,
+                                             short4_t& p0, short4_t& p1, short4_t& p2) {
+  const int G = g.G;
+  if (KIND == kConst) {
+    const uint16_t* const* parts = LEFT ? g.a_p : g.bt_p;
+    const size_t o = (size_t)idx_row * G + k;
+    p0 = __builtin_bit_cast(short4_t, *reinterpret_cast<const uint2*>(parts[0] + o));
+    p1 = __builtin_bit_cast(short4_t, *reinterpret_cast<const uint2*>(parts[1] + o));
+    if (SPLIT == 3) p2 = __builtin_bit_cast(short4_t, *reinterpret_cast<const uint2*>(parts[2] + o));
+  } else if (LEFT) {
+    const float4 v4 = *reinterpret_cast<const float4*>(g.a_f + (size_t)idx_row * G + k);
+    const float v[4] = {v4.x, v4.y, v4.z, v4.w};
+    split4<SPLIT>(v, p0, p1, p2);
+  } else {
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const size_t idx = (size_t)(k + j) * G + idx_row;
+      if (KIND == kCounts) v[j] = (float)g.b_cnt[idx] * g.cnt_w - g.mean;
+      else v[j] = g.b_f[idx];
+    }
+    split4<SPLIT>(v, p0, p1, p2);
+  }
+}
+
+#define MFMA(a, b) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0)
+
+template <int AK, int BK, int EPI, int SPLIT>
 __global__ __launch_bounds__(64) void dct_gemm(GemmArgs g) {
   const int G = g.G;
   const int tiles_n = G / 32;
   const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
   const int lane = threadIdx.x;
-  const int r = lane & 31;         // row of A fragment / col of B fragment
+  const int r = lane & 31;         // row of the A fragment / column of the B fragment
   const int kq = (lane >> 5) * 4;  // k offset inside the 8-wide step
   const int arow = tm * 32 + r;
   const int bcol = tn * 32 + r;
   float16_t acc = {};
+  short4_t a0, a1, a2, b0, b1, b2;
+  load_operand<AK, SPLIT, true>(g, arow, kq, a0, a1, a2);
+  load_operand<BK, SPLIT, false>(g, bcol, kq, b0, b1, b2);
   for (int k0 = 0; k0 < G; k0 += 8) {
-    short4_t ahi, alo, bhi, blo;
-    if (AK == kConst) {
-      const uint2 h = *reinterpret_cast<const uint2*>(g.a_hi + (size_t)arow * G + k0 + kq);
-      const uint2 l = *reinterpret_cast<const uint2*>(g.a_lo + (size_t)arow * G + k0 + kq);
-      ahi = __builtin_bit_cast(short4_t, h);
-      alo = __builtin_bit_cast(short4_t, l);
+    short4_t na0, na1, na2, nb0, nb1, nb2;
+    const int kn = k0 + 8 < G ? k0 + 8 : k0;  // prefetch the next step (last: reload)
+    load_operand<AK, SPLIT, true>(g, arow, kn + kq, na0, na1, na2);
+    load_operand<BK, SPLIT, false>(g, bcol, kn + kq, nb0, nb1, nb2);
+    if (SPLIT == 3) {
+      MFMA(a2, b0);
+      MFMA(a1, b1);
+      MFMA(a0, b2);
+      MFMA(a1, b0);
+      MFMA(a0, b1);
+      MFMA(a0, b0);
     } else {
-      const float4 v4 = *reinterpret_cast<const float4*>(g.a_f + (size_t)arow * G + k0 + kq);
-      const float v[4] = {v4.x, v4.y, v4.z, v4.w};
-      split4(v, ahi, alo);
+      if (g.passes == 4) MFMA(a1, b1);
+      MFMA(a1, b0);
+      MFMA(a0, b1);
+      MFMA(a0, b0);
     }
-    if (BK == kConst) {
-      const uint2 h = *reinterpret_cast<const uint2*>(g.bt_hi + (size_t)bcol * G + k0 + kq);
-      const uint2 l = *reinterpret_cast<const uint2*>(g.bt_lo + (size_t)bcol * G + k0 + kq);
-      bhi = __builtin_bit_cast(short4_t, h);
-      blo = __builtin_bit_cast(short4_t, l);
-    } else {
-      float v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const size_t idx = (size_t)(k0 + kq + j) * G + bcol;
-        if (BK == kCounts) v[j] = (float)g.b_cnt[idx] * g.cnt_w - g.target;
-        else v[j] = g.b_f[idx];
-      }
-      split4(v, bhi, blo);
-    }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ahi, bhi, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ahi, blo, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(alo, bhi, acc, 0, 0, 0);
-    if (g.passes == 4) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(alo, blo, acc, 0, 0, 0);
+    a0 = na0; a1 = na1; a2 = na2;
+    b0 = nb0; b1 = nb1; b2 = nb2;
   }
   // D layout: register e -> row (e/4)*8 + (lane/32)*4 + e%4, col lane%32.
 #pragma unroll
@@ -136,6 +172,7 @@ __global__ __launch_bounds__(64) void dct_gemm(GemmArgs g) {
     g.out[o] = v;
   }
 }
+#undef MFMA
 
 __global__ void deposit_kernel(int N, const float* __restrict__ x, const float* __restrict__ y,
                                double bin_w, double bin_h, int G, int* __restrict__ counts) {
@@ -176,12 +213,14 @@ int dct_alloc(Job& j) {
   int rc;
   fx_ctx* c = j.ctx;
   if ((rc = d.counts.alloc(c, 4 * GG)) || (rc = d.c_hi.alloc(c, 2 * GG)) ||
-      (rc = d.c_lo.alloc(c, 2 * GG)) || (rc = d.ct_hi.alloc(c, 2 * GG)) ||
+      (rc = d.c_mid.alloc(c, 2 * GG)) || (rc = d.c_lo.alloc(c, 2 * GG)) ||
+      (rc = d.ct_hi.alloc(c, 2 * GG)) || (rc = d.ct_mid.alloc(c, 2 * GG)) ||
       (rc = d.ct_lo.alloc(c, 2 * GG)) || (rc = d.t0.alloc(c, 4 * GG)) ||
       (rc = d.t1.alloc(c, 4 * GG)) || (rc = d.phi.alloc(c, 4 * GG)) ||
-      (rc = d.inv_eig.alloc(c, 4 * GG)))
+      (rc = d.eig.alloc(c, 4 * GG)))
     return rc;
-  std::vector<uint16_t> ch(GG), cl(GG), th(GG), tl(GG);
+  std::vector<uint16_t> parts[6];
+  for (auto& v : parts) v.resize(GG);
   std::vector<float> eig(GG);
   const double pi = 3.14159265358979323846;
   for (int k = 0; k < G; ++k) {
@@ -189,11 +228,16 @@ int dct_alloc(Job& j) {
     for (int m = 0; m < G; ++m) {
       const float v = (float)(s * std::cos(pi * k * (2 * m + 1) / (2.0 * G)));
       const uint16_t h = bf16_rne(v);
-      const uint16_t l = bf16_rne(v - bf16_to_f(h));
-      ch[(size_t)k * G + m] = h;
-      cl[(size_t)k * G + m] = l;
-      th[(size_t)m * G + k] = h;
-      tl[(size_t)m * G + k] = l;
+      const float r = v - bf16_to_f(h);
+      const uint16_t md = bf16_rne(r);
+      const uint16_t l = bf16_rne(r - bf16_to_f(md));
+      const size_t o = (size_t)k * G + m, t = (size_t)m * G + k;
+      parts[0][o] = h;
+      parts[1][o] = md;
+      parts[2][o] = l;
+      parts[3][t] = h;
+      parts[4][t] = md;
+      parts[5][t] = l;
     }
   }
   const double bin_w = (double)j.die_w / G, bin_h = (double)j.die_h / G;
@@ -206,11 +250,10 @@ int dct_alloc(Job& j) {
       if (kx == 0 && ky == 0) ev = 1.0;
       eig[(size_t)ky * G + kx] = (float)ev;
     }
-  FX_HIP(hipMemcpyAsync(d.c_hi.p, ch.data(), 2 * GG, hipMemcpyHostToDevice, j.stream));
-  FX_HIP(hipMemcpyAsync(d.c_lo.p, cl.data(), 2 * GG, hipMemcpyHostToDevice, j.stream));
-  FX_HIP(hipMemcpyAsync(d.ct_hi.p, th.data(), 2 * GG, hipMemcpyHostToDevice, j.stream));
-  FX_HIP(hipMemcpyAsync(d.ct_lo.p, tl.data(), 2 * GG, hipMemcpyHostToDevice, j.stream));
-  FX_HIP(hipMemcpyAsync(d.inv_eig.p, eig.data(), 4 * GG, hipMemcpyHostToDevice, j.stream));
+  DevBuf* dst[6] = {&d.c_hi, &d.c_mid, &d.c_lo, &d.ct_hi, &d.ct_mid, &d.ct_lo};
+  for (int k = 0; k < 6; ++k)
+    FX_HIP(hipMemcpyAsync(dst[k]->p, parts[k].data(), 2 * GG, hipMemcpyHostToDevice, j.stream));
+  FX_HIP(hipMemcpyAsync(d.eig.p, eig.data(), 4 * GG, hipMemcpyHostToDevice, j.stream));
   FX_HIP(hipStreamSynchronize(j.stream));
   return FX_OK;
 }
@@ -228,38 +271,43 @@ int dct_field(Job& j) {
   GemmArgs a{};
   a.G = G;
   a.passes = j.p.dct_passes;
-  a.cnt_w = (float)(1.0 / (bin_w * bin_h));
-  a.target = j.p.target_density;
-  a.eig = d.inv_eig.as<float>();
+  const double w = 1.0 / (bin_w * bin_h);
+  a.cnt_w = (float)w;
+  a.mean = (float)((double)j.N / ((double)G * G) * (double)a.cnt_w);
+  a.eig = d.eig.as<float>();
+  const uint16_t* C[3] = {d.c_hi.as<uint16_t>(), d.c_mid.as<uint16_t>(), d.c_lo.as<uint16_t>()};
+  const uint16_t* Ct[3] = {d.ct_hi.as<uint16_t>(), d.ct_mid.as<uint16_t>(), d.ct_lo.as<uint16_t>()};
   const dim3 grid((unsigned)((G / 32) * (G / 32))), block(64);
   // T1 = C * a
   GemmArgs g1 = a;
-  g1.a_hi = d.c_hi.as<uint16_t>();
-  g1.a_lo = d.c_lo.as<uint16_t>();
+  for (int k = 0; k < 3; ++k) g1.a_p[k] = C[k];
   g1.b_cnt = d.counts.as<int>();
   g1.out = d.t0.as<float>();
-  hipLaunchKernelGGL((dct_gemm<kConst, kCounts, kStore>), grid, block, 0, st, g1);
+  if (a.passes == 6)
+    hipLaunchKernelGGL((dct_gemm<kConst, kCounts, kStore, 3>), grid, block, 0, st, g1);
+  else
+    hipLaunchKernelGGL((dct_gemm<kConst, kCounts, kStore, 2>), grid, block, 0, st, g1);
   // phi_hat = -(T1 * C^T) / lambda   (B = C^T is stored transposed: C)
   GemmArgs g2 = a;
   g2.a_f = d.t0.as<float>();
-  g2.bt_hi = d.c_hi.as<uint16_t>();
-  g2.bt_lo = d.c_lo.as<uint16_t>();
+  for (int k = 0; k < 3; ++k) g2.bt_p[k] = C[k];
   g2.out = d.t1.as<float>();
-  hipLaunchKernelGGL((dct_gemm<kData, kConst, kEigen>), grid, block, 0, st, g2);
+  if (a.passes == 6)
+    hipLaunchKernelGGL((dct_gemm<kData, kConst, kEigen, 3>), grid, block, 0, st, g2);
+  else
+    hipLaunchKernelGGL((dct_gemm<kData, kConst, kEigen, 2>), grid, block, 0, st, g2);
   // T2 = C^T * phi_hat
   GemmArgs g3 = a;
-  g3.a_hi = d.ct_hi.as<uint16_t>();
-  g3.a_lo = d.ct_lo.as<uint16_t>();
+  for (int k = 0; k < 3; ++k) g3.a_p[k] = Ct[k];
   g3.b_f = d.t1.as<float>();
   g3.out = d.t0.as<float>();
-  hipLaunchKernelGGL((dct_gemm<kConst, kData, kStore>), grid, block, 0, st, g3);
+  hipLaunchKernelGGL((dct_gemm<kConst, kData, kStore, 3>), grid, block, 0, st, g3);
   // phi = T2 * C   (B = C is stored transposed: C^T)
   GemmArgs g4 = a;
   g4.a_f = d.t0.as<float>();
-  g4.bt_hi = d.ct_hi.as<uint16_t>();
-  g4.bt_lo = d.ct_lo.as<uint16_t>();
+  for (int k = 0; k < 3; ++k) g4.bt_p[k] = Ct[k];
   g4.out = d.phi.as<float>();
-  hipLaunchKernelGGL((dct_gemm<kData, kConst, kStore>), grid, block, 0, st, g4);
+  hipLaunchKernelGGL((dct_gemm<kData, kConst, kStore, 3>), grid, block, 0, st, g4);
   FX_HIP(hipGetLastError());
   return FX_OK;
 }

@@ -652,10 +652,10 @@ extern "C" int fx_job_create(fx_ctx* ctx, const fx_netlist* net, const fx_params
     FX_CHECK(net->net_pins[p] >= 0 && net->net_pins[p] < net->num_gates, FX_E_INVALID,
              "pin %d: gate id %d out of range", p, net->net_pins[p]);
   const int G = params->density_grid;
-  FX_CHECK(G == 0 || (G >= 16 && G <= 2048 && G % 32 == 0), FX_E_INVALID,
+  FX_CHECK(G == 0 || (G >= 32 && G <= 2048 && G % 32 == 0), FX_E_INVALID,
            "density_grid %d: must be 0 or a multiple of 32 in [32, 2048]", G);
-  FX_CHECK(params->dct_passes == 3 || params->dct_passes == 4, FX_E_INVALID,
-           "dct_passes must be 3 or 4");
+  FX_CHECK(params->dct_passes == 3 || params->dct_passes == 4 || params->dct_passes == 6,
+           FX_E_INVALID, "dct_passes must be 3, 4 or 6");
   FX_HIP(hipSetDevice(ctx->device));
   fx_job* job = new fx_job();
   Job& j = job->j;

@@ -39,9 +39,10 @@ struct BhState {
 struct DctState {
   int G = 0;
   DevBuf counts;                 // int32 G*G point-in-bin counts
-  DevBuf c_hi, c_lo, ct_hi, ct_lo;  // bf16 splits of C and C^T
+  // bf16 splits of C and C^T: x = hi + mid + lo (mid = bf16(x - hi))
+  DevBuf c_hi, c_mid, c_lo, ct_hi, ct_mid, ct_lo;
   DevBuf t0, t1, phi;            // f32 G*G work arrays
-  DevBuf inv_eig;                // f32 G*G: -1/lambda (0 at DC)
+  DevBuf eig;                    // f32 G*G eigenvalues lambda (1 at DC)
 };
 
 struct Job {

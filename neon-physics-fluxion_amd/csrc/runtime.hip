@@ -64,7 +64,7 @@ extern "C" void fx_params_default(fx_params* p) {
   p->density_weight = 1.0f;  // force_electrostatic.py:36
   p->target_density = 0.8f;  // force_electrostatic.py:37
   p->density_grid = 0;
-  p->dct_passes = 3;
+  p->dct_passes = 6;
   p->flags = 0;
 }
 

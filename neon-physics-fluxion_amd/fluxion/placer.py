@@ -48,7 +48,7 @@ class PlacerConfig:
     density_weight: float = 1.0
     target_density: float = 0.8
     density_grid: int = 0
-    dct_passes: int = 3
+    dct_passes: int = 6
     springs: bool = True
     repulsion: bool = True
     density: bool = True

@@ -20,6 +20,10 @@ def make(N, E, seed, **cfg):
 
 
 fxn_ctx = [None]
+# Density weight of the flat workloads: the reference eigenvalue grows as k^4, so at
+# weight 1.0 the explicit step is unstable (all gates end on a die corner, CPU path
+# included); 1e-4 is the stable setting measured at 200k gates / G=512.
+DW = 1e-4
 
 
 @pytest.fixture(autouse=True)
@@ -135,29 +139,29 @@ def test_density_fields_vs_fp64(G):
 def test_full_step_teacher_forced():
     print("case: flat step teacher forced", flush=True)
     N, G = 20000, 128
-    circ, x, y, job = make(N, 22000, 61, density_grid=G)
+    circ, x, y, job = make(N, 22000, 61, density_grid=G, density_weight=DW)
     ox, oy, ovx, ovy = x, y, np.zeros(N, np.float32), np.zeros(N, np.float32)
     for t in range(4):
         job.set_positions(ox, oy)
         job.set_velocities(ovx, ovy)
         gx, gy = job.forces(7)
         *_, rfx, rfy = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, ox, oy, ovx, ovy,
-                                    iters=0, G=G, nthreads=8)
+                                    iters=0, G=G, dens_weight=DW, nthreads=8)
         err = max(nrm_err(gx, rfx), nrm_err(gy, rfy))
         print(f"step {t}: force err {err:.2e}", flush=True)
         assert err < 1e-4
         ox, oy, ovx, ovy, *_ = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, ox, oy,
-                                            ovx, ovy, iters=25, G=G, nthreads=8)
+                                            ovx, ovy, iters=25, G=G, dens_weight=DW, nthreads=8)
 
 
 def test_run_hpwl_vs_fp64_and_graph_identical():
     print("case: flat run vs fp64", flush=True)
     N, G, T = 4000, 64, 100
-    circ, x, y, job = make(N, 4400, 71, density_grid=G)
+    circ, x, y, job = make(N, 4400, 71, density_grid=G, density_weight=DW)
     job.run(T)
     gx, gy = job.positions()
     h = job.hpwl()
-    ox, oy, *_ = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, x, y, iters=T, G=G,
+    ox, oy, *_ = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, x, y, iters=T, G=G, dens_weight=DW,
                               dtype=np.float64)
     h64 = orc.hpwl(circ.net_ptr, circ.net_pins, ox, oy)
     print(f"HPWL gpu {h:.2f} fp64 {h64:.2f} rel {abs(h - h64) / h64:.2e}", flush=True)
@@ -168,7 +172,7 @@ def test_run_hpwl_vs_fp64_and_graph_identical():
     rt, rm = orc.overflow(c64, 1000.0, 1000.0)
     print(f"overflow gpu {ot:.1f} fp64-run {rt:.1f}", flush=True)
     # graph replay == eager, bit for bit
-    job2 = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(density_grid=G))
+    job2 = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(density_grid=G, density_weight=DW))
     job2.set_positions(x, y)
     job2.use_graph(True)
     for _ in range(4):

@@ -251,3 +251,30 @@ def test_springs_centroid_equals_clique_pairs():
                     ey[i] += y[j] - y[i]
     np.testing.assert_allclose(fx, ex, rtol=1e-10, atol=1e-9)
     np.testing.assert_allclose(fy, ey, rtol=1e-10, atol=1e-9)
+
+
+def test_density_oracle_matches_scipy_golden():
+    # Golden vectors from scipy.fft.dctn/idctn (tests/golden/make_dct_golden.py).
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dct_golden.npz"))
+    for G in (8, 16, 64):
+        die = float(g[f"G{G}_die"])
+        fx, fy, rho, phi, ex, ey = orc.density(g[f"G{G}_x"], g[f"G{G}_y"], die, die, G, fields=True)
+        for name, a in (("rho", rho), ("phi", phi), ("ex", ex), ("ey", ey), ("fx", fx), ("fy", fy)):
+            ref = g[f"G{G}_{name}"]
+            assert np.abs(a - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), (G, name)
+
+
+def test_density_linearity_and_dc_removal():
+    # The Poisson solve is linear in the density, and a uniform background
+    # (the target density) only moves the DC term, which is zeroed.
+    G, die = 16, 160.0
+    rng = np.random.default_rng(11)
+    xa, ya = rng.uniform(0, die, 400).astype(np.float32), rng.uniform(0, die, 400).astype(np.float32)
+    xb, yb = rng.uniform(0, die, 300).astype(np.float32), rng.uniform(0, die, 300).astype(np.float32)
+    _, _, _, pa, _, _ = orc.density(xa, ya, die, die, G, target=0.0, fields=True)
+    _, _, _, pb, _, _ = orc.density(xb, yb, die, die, G, target=0.0, fields=True)
+    _, _, _, pab, _, _ = orc.density(np.r_[xa, xb], np.r_[ya, yb], die, die, G, target=0.0, fields=True)
+    np.testing.assert_allclose(pab, pa + pb, atol=1e-9 * np.abs(pab).max())
+    _, _, _, p1, _, _ = orc.density(xa, ya, die, die, G, target=0.8, fields=True)
+    np.testing.assert_allclose(pa, p1, atol=1e-9 * np.abs(pa).max())
