@@ -29,7 +29,9 @@
 // The walk visits accepted cells in the reference's order, so each
 // particle's force is summed in the same order as the CPU path of record.
 
+#include <climits>
 #include <cmath>
+#include <cstdlib>
 
 #include "job.h"
 
@@ -83,14 +85,22 @@ __device__ __forceinline__ int bitlen_u64(unsigned long long v) {
 // Root cell (BarnesHutTree.cpp:24-46), merge depth and fixed-point scale.
 __global__ void bh_geometry(const float4* __restrict__ part, int nparts, int n,
                             BhGeometry* __restrict__ g) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  float4 r = part[0];
-  for (int k = 1; k < nparts; ++k) {
+  // One wave: lanes reduce strided partials, then a shuffle reduction.
+  const int lane = threadIdx.x;
+  float4 r = part[lane < nparts ? lane : 0];
+  for (int k = lane + 64; k < nparts; k += 64) {
     r.x = fminf(r.x, part[k].x);
     r.y = fmaxf(r.y, part[k].y);
     r.z = fminf(r.z, part[k].z);
     r.w = fmaxf(r.w, part[k].w);
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    r.x = fminf(r.x, __shfl_down(r.x, off, 64));
+    r.y = fmaxf(r.y, __shfl_down(r.y, off, 64));
+    r.z = fminf(r.z, __shfl_down(r.z, off, 64));
+    r.w = fmaxf(r.w, __shfl_down(r.w, off, 64));
+  }
+  if (lane != 0) return;
   const double min_x = (double)r.x - 1.0, max_x = (double)r.y + 1.0;
   const double min_y = (double)r.z - 1.0, max_y = (double)r.w + 1.0;
   const double size = fmax(max_x - min_x, max_y - min_y);
@@ -161,7 +171,7 @@ __device__ __forceinline__ int lcp_levels(const uint64_t* __restrict__ keys, int
 __global__ __launch_bounds__(kBlock) void bh_lcp_count(
     int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const float* __restrict__ x, const float* __restrict__ y, const BhGeometry* __restrict__ g,
-    int8_t* __restrict__ L, int* __restrict__ bmin1, int* __restrict__ bmin2,
+    int8_t* __restrict__ L, int8_t* __restrict__ bmin1, int* __restrict__ bmin2,
     int* __restrict__ owned, I64x2* __restrict__ fix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int D = g->depth;
@@ -184,42 +194,78 @@ __global__ __launch_bounds__(kBlock) void bh_lcp_count(
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kBlock / 64; ++w) m = min(m, sh[w]);
-    bmin1[blockIdx.x] = m;
+    bmin1[blockIdx.x] = (int8_t)m;
     atomicMin(&bmin2[blockIdx.x >> 8], m);
   }
 }
 
-// Last sorted position of the cell of depth d that starts at position i:
-// the first q >= i with L[q] < d (L[n-1] = -1 ends every search).
-__device__ int find_end(const int8_t* __restrict__ L, const int* __restrict__ b1,
-                        const int* __restrict__ b2, int n, int nb1, int nb2, int i, int d) {
-  int q = i;
-  const int blk_end = min(n, ((q >> 8) + 1) << 8);
-  for (; q < blk_end; ++q)
-    if (L[q] < d) return q;
-  int b = q >> 8;
-  const int sb_end = min(nb1, ((b >> 8) + 1) << 8);
-  for (; b < sb_end; ++b)
-    if (b1[b] < d) break;
-  if (b == sb_end) {
-    int s = b >> 8;
-    for (; s < nb2; ++s)
-      if (b2[s] < d) break;
-    b = s << 8;
-    for (; b < nb1; ++b)
-      if (b1[b] < d) break;
+// First index q in [s, e) with a[q] < d, else e. Reads 16 bytes at a time
+// once aligned (arrays are padded by 16 bytes).
+__device__ __forceinline__ int first_lt(const int8_t* __restrict__ a, int s, int e, int d) {
+  int q = s;
+  for (; q < e && (q & 15); ++q)
+    if (a[q] < d) return q;
+  for (; q < e; q += 16) {
+    const uint4 v = *reinterpret_cast<const uint4*>(a + q);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int b = (int)(int8_t)((w[k >> 2] >> (8 * (k & 3))) & 0xFFu);
+      if (b < d && q + k < e) return q + k;
+    }
   }
-  for (q = b << 8; q < n; ++q)
-    if (L[q] < d) return q;
-  return n - 1;
+  return e;
 }
 
-__global__ void bh_emit(int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                        const int8_t* __restrict__ L, const int* __restrict__ bmin1,
-                        const int* __restrict__ bmin2, int nb1, int nb2,
-                        const int* __restrict__ offset, const I64x2* __restrict__ fixpre,
-                        BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap) {
+// First position q >= pos with L[q] < d (L[n-1] = -1 ends every search).
+// Inside the workgroup's own 256 positions (block blk) the answer comes from
+// per-depth ballot masks; beyond it, from the block minima (256 positions
+// each) and the super-block minima (256 blocks each), 16 bytes per load.
+__device__ int find_lt(const int8_t* __restrict__ L, const int8_t* __restrict__ b1,
+                       const int* __restrict__ b2, int n, int nb1, int nb2,
+                       const unsigned long long (*masks)[33], int blk, int pos, int d) {
+  int b;
+  if ((pos >> 8) == blk) {
+    const int wave = (pos & 255) >> 6, lane = pos & 63;
+    const unsigned long long m = masks[wave][d] & (~0ull << lane);
+    if (m) return (pos & ~63) + __ffsll((long long)m) - 1;
+    for (int w = wave + 1; w < 4; ++w)
+      if (masks[w][d]) return (pos & ~255) + w * 64 + __ffsll((long long)masks[w][d]) - 1;
+    b = blk + 1;
+  } else {
+    const int e = min(n, ((pos >> 8) + 1) << 8);
+    const int q = first_lt(L, pos, e, d);
+    if (q < e) return q;
+    b = (pos >> 8) + 1;
+  }
+  if (b >= nb1) return n - 1;
+  const int sb_end = min(nb1, ((b >> 8) + 1) << 8);
+  b = first_lt(b1, b, sb_end, d);
+  if (b == sb_end) {
+    int s = b >> 8;
+    while (s < nb2 && b2[s] >= d) ++s;
+    if (s >= nb2) return n - 1;
+    b = first_lt(b1, s << 8, min(nb1, (s + 1) << 8), d);
+  }
+  if (b >= nb1) return n - 1;
+  const int q = first_lt(L, b << 8, min(n, (b + 1) << 8), d);
+  return q < n ? q : n - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void bh_emit(
+    int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    const int8_t* __restrict__ L, const int8_t* __restrict__ bmin1, const int* __restrict__ bmin2,
+    int nb1, int nb2, const int* __restrict__ offset, const I64x2* __restrict__ fixpre,
+    BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap) {
+  __shared__ unsigned long long masks[kBlock / 64][33];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = i < n ? (int)L[i] : -1;
+  for (int d = 0; d <= 32; ++d) {
+    const unsigned long long m = __ballot(li < d);
+    if (lane == 0) masks[wave][d] = m;
+  }
+  __syncthreads();
   if (i >= n) return;
   const int D = g->depth;
   const int M = offset[n];
@@ -228,17 +274,27 @@ __global__ void bh_emit(int n, const uint64_t* __restrict__ keys, const uint32_t
     if ((int64_t)M > cap) g->overflow = 1;
   }
   const int lp = i > 0 ? (int)L[i - 1] : -1;
-  const int li = L[i];
   if (lp >= D) return;  // continuation of a merged leaf
   const int internal = max(0, min(li, D - 1) - lp);
   const int id0 = offset[i];
   const uint64_t key = keys[i];
   const double r0 = g->root[0], r1 = g->root[1], scale = g->scale;
+  // The cells starting here are nested: ends are found deepest first, each
+  // search starting at the end of the cell below it.
+  const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
+  const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
+  int ends[33];
+  {
+    int e = find_lt(L, bmin1, bmin2, n, nb1, nb2, masks, blockIdx.x, i, leaf_depth);
+    ends[internal] = e;
+    for (int k = internal - 1; k >= 0; --k) {
+      e = find_lt(L, bmin1, bmin2, n, nb1, nb2, masks, blockIdx.x, e, first_depth + k);
+      ends[k] = e;
+    }
+  }
   // Walk down the key once, emitting the cells that start here.
   double xl = g->root[0], yl = g->root[1], xh = g->root[2], yh = g->root[3];
   int depth = 0;
-  const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
-  const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
   for (int k = 0; k <= internal; ++k) {
     const bool is_leaf = (k == internal);
     const int d = is_leaf ? leaf_depth : first_depth + k;
@@ -251,7 +307,7 @@ __global__ void bh_emit(int n, const uint64_t* __restrict__ keys, const uint32_t
     }
     const int id = id0 + k;
     if ((int64_t)id >= cap) continue;
-    const int end = find_end(L, bmin1, bmin2, n, nb1, nb2, i, d);
+    const int end = ends[k];
     const int count = end - i + 1;
     const long long sx = fixpre[end + 1].a - fixpre[i].a;
     const long long sy = fixpre[end + 1].b - fixpre[i].b;
@@ -268,54 +324,130 @@ __global__ void bh_emit(int n, const uint64_t* __restrict__ keys, const uint32_t
   }
 }
 
-template <bool kStats>
+// Walk of one particle, split over the root's children. W consecutive lanes
+// (W = 1, 2 or 4) share a particle; each evaluates the root
+// (BarnesHutTree.cpp:148-160) and, when it is opened, lane r walks the
+// subtrees of the root's existing children r, r+W, ... — each a contiguous
+// range of the pre-order array — with the stackless loop: skip the leaf
+// holding the particle (149), accept a leaf or a cell with size/dist < theta
+// (157), else step to the next node (first child). One partial sum per
+// quadrant; the four are added in quadrant order, as the path of record
+// does, so every W gives the same bits. More lanes per particle put more
+// waves in flight to hide the latency of the dependent node loads.
+template <bool kStats, int W>
 __global__ __launch_bounds__(kBlock) void bh_traverse(
     int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
     const float* __restrict__ y, const BhGeometry* __restrict__ g,
     const BhNode* __restrict__ nodes, float k, float theta, float min_dist,
     float* __restrict__ fx, float* __restrict__ fy, unsigned long long* __restrict__ stats) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int p = (int)vals[t];
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = gid / W, r = gid % W;
+  const bool valid = t < n;
+  const int p = valid ? (int)vals[t] : -1;
   if (g->overflow) {
-    fx[p] = 0.0f;
-    fy[p] = 0.0f;
+    if (valid && r == 0) {
+      fx[p] = 0.0f;
+      fy[p] = 0.0f;
+    }
     return;
   }
-  const int M = g->num_nodes;
-  const float px = x[p], py = y[p];
   const float charge = 1.0f;
-  float ax = 0.0f, ay = 0.0f;
+  float qx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, qy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   unsigned long long inter = 0, opened = 0;
-  int cur = 0;
-  while (cur < M) {
-    const BhNode nd = nodes[cur];
-    if (nd.skip <= cur) break;  // malformed tree: never loop
-    if (nd.leaf == p) {
-      cur = nd.skip;
-      continue;
+  if (valid) {
+    const float px = x[p], py = y[p];
+    const BhNode root = nodes[0];
+    bool split = false;
+    if (root.leaf != p) {
+      const float dx = px - root.cx, dy = py - root.cy;
+      const float dist_sq = dx * dx + dy * dy;
+      float dist = sqrtf(dist_sq);
+      if (dist < min_dist) dist = min_dist;
+      if (root.leaf >= 0 || (root.size / dist < theta)) {
+        if (r == 0) {
+          const float f = (k * charge * root.q) / (dist * dist);
+          qx[0] += f * (dx / dist);
+          qy[0] += f * (dy / dist);
+          if (kStats) ++inter;
+        }
+      } else {
+        split = true;
+        if (kStats && r == 0) ++opened;
+      }
     }
-    const float dx = px - nd.cx, dy = py - nd.cy;
-    const float dist_sq = dx * dx + dy * dy;
-    float dist = sqrtf(dist_sq);
-    if (dist < min_dist) dist = min_dist;
-    if (nd.leaf >= 0 || (nd.size / dist < theta)) {
-      const float f = (k * charge * nd.q) / (dist * dist);
-      ax += f * (dx / dist);
-      ay += f * (dy / dist);
-      cur = nd.skip;
-      if (kStats) ++inter;
-    } else {
-      cur = cur + 1;
-      if (kStats) ++opened;
+    if (split) {
+      int c = 1;
+      for (int j = 0; j < r && c < root.skip; ++j) c = nodes[c].skip;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q % W != r) continue;
+        if (c >= root.skip) break;
+        int cur = c;
+        const int end = nodes[c].skip;
+        float ax = 0.0f, ay = 0.0f;
+        while (cur < end) {
+          const BhNode nd = nodes[cur];
+          if (nd.skip <= cur) break;  // malformed tree: never loop
+          if (nd.leaf == p) {
+            cur = nd.skip;
+            continue;
+          }
+          const float dx = px - nd.cx, dy = py - nd.cy;
+          const float dist_sq = dx * dx + dy * dy;
+          float dist = sqrtf(dist_sq);
+          if (dist < min_dist) dist = min_dist;
+          if (nd.leaf >= 0 || (nd.size / dist < theta)) {
+            const float f = (k * charge * nd.q) / (dist * dist);
+            ax += f * (dx / dist);
+            ay += f * (dy / dist);
+            cur = nd.skip;
+            if (kStats) ++inter;
+          } else {
+            cur = cur + 1;
+            if (kStats) ++opened;
+          }
+        }
+        qx[q] = ax;
+        qy[q] = ay;
+        // advance to this lane's next quadrant (W siblings further)
+        for (int j = 0; j < W && c < root.skip; ++j) c = nodes[c].skip;
+      }
     }
   }
-  fx[p] = ax;
-  fy[p] = ay;
-  if (kStats) {
+  if (W > 1) {
+    const int base = (threadIdx.x & 63) - r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      qx[q] = __shfl(qx[q], base + q % W, 64);
+      qy[q] = __shfl(qy[q], base + q % W, 64);
+    }
+  }
+  if (valid && r == 0) {
+    fx[p] = ((qx[0] + qx[1]) + qx[2]) + qx[3];
+    fy[p] = ((qy[0] + qy[1]) + qy[2]) + qy[3];
+  }
+  if (kStats && valid) {
     atomicAdd(&stats[0], inter);
     atomicAdd(&stats[1], opened);
   }
+}
+
+template <bool kStats>
+int launch_walk(Job& j, unsigned long long* stats) {
+  BhState& b = j.bh;
+  const int W = b.walk_lanes;
+  const dim3 grid(div_up((int64_t)W * j.N, kBlock)), block(kBlock);
+#define FX_WALK(WW)                                                                              \
+  hipLaunchKernelGGL((bh_traverse<kStats, WW>), grid, block, 0, j.stream, j.N,                   \
+                     b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),                    \
+                     (const BhGeometry*)b.geom.p, b.nodes.as<BhNode>(), j.p.repel_weight,        \
+                     j.p.theta, j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(), stats)
+  if (W == 1) FX_WALK(1);
+  else if (W == 2) FX_WALK(2);
+  else FX_WALK(4);
+#undef FX_WALK
+  FX_HIP(hipGetLastError());
+  return FX_OK;
 }
 
 }  // namespace
@@ -332,12 +464,19 @@ int bh_alloc(Job& j) {
   if ((rc = b.part.alloc(c, sizeof(float4) * kReduceBlocks)) ||
       (rc = b.geom.alloc(c, sizeof(BhGeometry))) || (rc = b.keys.alloc(c, 8 * n)) ||
       (rc = b.vals.alloc(c, 4 * n)) || (rc = b.lcp.alloc(c, n + 16)) ||
-      (rc = b.bmin1.alloc(c, 4 * b.nb1)) || (rc = b.bmin2.alloc(c, 4 * b.nb2)) ||
+      (rc = b.bmin1.alloc(c, b.nb1 + 32)) || (rc = b.bmin2.alloc(c, 4 * b.nb2)) ||
       (rc = b.owned.alloc(c, 4 * n)) || (rc = b.offset.alloc(c, 4 * (n + 1))) ||
       (rc = b.sxp.alloc(c, 16 * n)) || (rc = b.syp.alloc(c, 16 * (n + 1) + 16 * scan_elems)) ||
       (rc = b.fx.alloc(c, 4 * n)) || (rc = b.fy.alloc(c, 4 * n)) ||
       (rc = b.stats.alloc(c, 64)) || (rc = b.sort.ensure(c, n)))
     return rc;
+  // Lanes per particle in the walk. Measured on MI355X over the 500-iteration
+  // 200k-gate run: W=1 0.185 ms, W=2 0.206 ms, W=4 0.356 ms per walk.
+  b.walk_lanes = 1;
+  if (const char* e = getenv("FLUXION_WALK_LANES")) {
+    const int w = atoi(e);
+    if (w == 1 || w == 2 || w == 4) b.walk_lanes = w;
+  }
   return bh_grow(j, std::max<int64_t>(3 * n, n + 64 * 33) + 1024);
 }
 
@@ -374,7 +513,7 @@ static int bh_build(Job& j) {
   if ((rc = phase_end(j, kPhaseBhSort)) || (rc = phase_begin(j, kPhaseBhTree))) return rc;
   hipLaunchKernelGGL(bh_lcp_count, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
-                     (const BhGeometry*)g, b.lcp.as<int8_t>(), b.bmin1.as<int>(), b.bmin2.as<int>(),
+                     (const BhGeometry*)g, b.lcp.as<int8_t>(), b.bmin1.as<int8_t>(), b.bmin2.as<int>(),
                      b.owned.as<int>(), b.sxp.as<I64x2>());
   FX_HIP(hipGetLastError());
   const int64_t scan_elems = scan_tmp_elems(n + 1);
@@ -387,7 +526,7 @@ static int bh_build(Job& j) {
   if ((rc = scan_exclusive_i64x2(st, b.sxp.p, fixpre, n, tmp, scan_elems))) return rc;
   hipLaunchKernelGGL(bh_emit, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), b.lcp.as<int8_t>(),
-                     b.bmin1.as<int>(), b.bmin2.as<int>(), (int)b.nb1, (int)b.nb2,
+                     b.bmin1.as<int8_t>(), b.bmin2.as<int>(), (int)b.nb1, (int)b.nb2,
                      b.offset.as<int>(), (const I64x2*)fixpre, g, b.nodes.as<BhNode>(), b.node_cap);
   FX_HIP(hipGetLastError());
   return phase_end(j, kPhaseBhTree);
@@ -398,11 +537,8 @@ int bh_forces(Job& j) {
   if (rc) return rc;
   BhState& b = j.bh;
   if ((rc = phase_begin(j, kPhaseBhWalk))) return rc;
-  hipLaunchKernelGGL(bh_traverse<false>, dim3(div_up(j.N, kBlock)), dim3(kBlock), 0, j.stream, j.N,
-                     b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
-                     (const BhGeometry*)b.geom.p, b.nodes.as<BhNode>(), j.p.repel_weight, j.p.theta,
-                     j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(), (unsigned long long*)nullptr);
-  FX_HIP(hipGetLastError());
+  if ((rc = launch_walk<false>(j, nullptr))) return rc;
+  b.sorted_valid = true;
   return phase_end(j, kPhaseBhWalk);
 }
 
@@ -418,12 +554,7 @@ extern "C" int fx_job_bh_stats(fx_job* job, int64_t* stats) {
     FX_HIP(hipMemsetAsync(b.stats.p, 0, 64, j.stream));
     int rc = fx::bh_build(j);
     if (rc) return rc;
-    hipLaunchKernelGGL(fx::bh_traverse<true>, dim3(fx::div_up(j.N, fx::kBlock)), dim3(fx::kBlock), 0,
-                       j.stream, j.N, b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
-                       (const fx::BhGeometry*)b.geom.p, b.nodes.as<fx::BhNode>(), j.p.repel_weight,
-                       j.p.theta, j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(),
-                       b.stats.as<unsigned long long>());
-    FX_HIP(hipGetLastError());
+    if ((rc = fx::launch_walk<true>(j, b.stats.as<unsigned long long>()))) return rc;
     fx::BhGeometry g;
     unsigned long long st[2];
     FX_HIP(hipMemcpyAsync(&g, b.geom.p, sizeof(g), hipMemcpyDeviceToHost, j.stream));

@@ -48,21 +48,17 @@ __global__ void net_centroid_kernel(int E, const int* __restrict__ net_ptr,
   netc[e] = make_float4(sx / deg, sy / deg, (t - s >= 2) ? w * deg : 0.0f, 0.0f);
 }
 
-// Density field at a gate's bin: E = -grad(phi) by the centred differences
-// of force_electrostatic.py:93-103, evaluated from phi on the fly.
-__device__ __forceinline__ void density_at(const float* __restrict__ phi, int G, double bin_w,
-                                           double bin_h, float wx, float wy, float xi, float yi,
-                                           float& ex, float& ey) {
+// Density field at a gate's bin (nearest-bin gather, force_electrostatic.py:
+// 134-145); the bin is computed in FP64 exactly as the reference's numpy code.
+__device__ __forceinline__ void density_at(const float* __restrict__ ex,
+                                           const float* __restrict__ ey, int G, double bin_w,
+                                           double bin_h, float xi, float yi, float& fx,
+                                           float& fy) {
   const double qc = fmin(fmax((double)xi / bin_w, 0.0), (double)(G - 1));
   const double qr = fmin(fmax((double)yi / bin_h, 0.0), (double)(G - 1));
-  const int c = (int)qc, r = (int)qr;
-  const float* row = phi + (size_t)r * G;
-  if (c == 0) ex = -(row[1] - row[0]) * wx;
-  else if (c == G - 1) ex = -(row[c] - row[c - 1]) * wx;
-  else ex = -(row[c + 1] - row[c - 1]) * wx * 0.5f;
-  if (r == 0) ey = -(phi[(size_t)G + c] - phi[c]) * wy;
-  else if (r == G - 1) ey = -(row[c] - phi[(size_t)(r - 1) * G + c]) * wy;
-  else ey = -(phi[(size_t)(r + 1) * G + c] - phi[(size_t)(r - 1) * G + c]) * wy * 0.5f;
+  const size_t b = (size_t)(int)qr * G + (int)qc;
+  fx = ex[b];
+  fy = ey[b];
 }
 
 struct CombineArgs {
@@ -72,10 +68,11 @@ struct CombineArgs {
   const float4* netc;
   const float* bfx;
   const float* bfy;
-  const float* phi;
+  const float* ex;
+  const float* ey;
   int G;
   double bin_w, bin_h;
-  float wx, wy, dens_w;
+  float dens_w;
   int comps;
   int integrate;
   float dt, damping, die_w, die_h;
@@ -110,7 +107,7 @@ __global__ void combine_kernel(CombineArgs a) {
   float dx = 0.0f, dy = 0.0f;
   if (a.comps & 4) {
     float ex, ey;
-    density_at(a.phi, a.G, a.bin_w, a.bin_h, a.wx, a.wy, xi, yi, ex, ey);
+    density_at(a.ex, a.ey, a.G, a.bin_w, a.bin_h, xi, yi, ex, ey);
     dx = a.dens_w * ex;
     dy = a.dens_w * ey;
   }
@@ -392,12 +389,11 @@ int combine(Job& j, int comps, bool integrate) {
   a.netc = j.netc.as<float4>();
   a.bfx = j.bh.fx.as<float>();
   a.bfy = j.bh.fy.as<float>();
-  a.phi = j.dct.phi.as<float>();
+  a.ex = j.dct.ex.as<float>();
+  a.ey = j.dct.ey.as<float>();
   a.G = j.dct.G;
   a.bin_w = j.dct.G ? (double)j.die_w / j.dct.G : 1.0;
   a.bin_h = j.dct.G ? (double)j.die_h / j.dct.G : 1.0;
-  a.wx = (float)(1.0 / a.bin_w);
-  a.wy = (float)(1.0 / a.bin_h);
   a.dens_w = j.p.density_weight;
   a.comps = comps;
   a.integrate = integrate ? 1 : 0;

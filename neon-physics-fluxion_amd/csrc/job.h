@@ -32,6 +32,8 @@ struct BhState {
   DevBuf part, geom, keys, vals, lcp, bmin1, bmin2, owned, offset, sxp, syp, nodes, fx, fy,
       stats;
   SortWorkspace sort;
+  bool sorted_valid = false;  // vals holds a permutation from a completed build
+  int walk_lanes = 1;         // lanes per particle in the tree walk (1, 2, 4)
   int64_t node_cap = 0;
   int64_t nb1 = 0, nb2 = 0;
 };
@@ -41,7 +43,12 @@ struct DctState {
   DevBuf counts;                 // int32 G*G point-in-bin counts
   // bf16 splits of C and C^T: x = hi + mid + lo (mid = bf16(x - hi))
   DevBuf c_hi, c_mid, c_lo, ct_hi, ct_mid, ct_lo;
-  DevBuf t0, t1, phi;            // f32 G*G work arrays
+  // bf16 splits of the differentiated bases: dxt[c][k] = sum_m C[k][m] Dx[m][c]
+  // and dy[r][k] = sum_m Dy[r][m] C[k][m] (Dx, Dy: the centred-difference
+  // operators of force_electrostatic.py:93-103 with their one-sided edges)
+  DevBuf dxt_hi, dxt_mid, dxt_lo, dy_hi, dy_mid, dy_lo;
+  DevBuf t0, t1, t2, phi;        // f32 G*G work arrays
+  DevBuf ex, ey;                 // f32 G*G field E = -grad(phi)
   DevBuf eig;                    // f32 G*G eigenvalues lambda (1 at DC)
 };
 

@@ -13,11 +13,11 @@
 namespace fx {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;
+constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;
 
 __global__ __launch_bounds__(kSortThreads) void radix_upsweep(const uint64_t* __restrict__ keys,
-                                                              int64_t n, int shift, int nblocks,
+                                                              int64_t n, int shift,
                                                               uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
@@ -29,40 +29,60 @@ __global__ __launch_bounds__(kSortThreads) void radix_upsweep(const uint64_t* __
     if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFF], 1u);
   }
   __syncthreads();
-  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+  hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of m uint32 values by one 1024-thread workgroup.
-__global__ __launch_bounds__(1024) void scan_single_block_u32(uint32_t* __restrict__ a, int64_t m) {
-  __shared__ uint32_t part[1024];
-  const int64_t per = (m + 1023) / 1024;
-  const int64_t s = threadIdx.x * per, e = min(m, s + per);
-  uint32_t sum = 0;
-  for (int64_t i = s; i < e; ++i) sum += a[i];
-  part[threadIdx.x] = sum;
+// Exclusive prefix over the tiles of every digit, in place, then the
+// exclusive prefix of the digit totals into dbase. 1024 threads: digit d is
+// scanned by four threads, each over a quarter of the tiles (loads coalesced
+// across digits), joined through LDS.
+__global__ __launch_bounds__(1024) void radix_scan(uint32_t* __restrict__ hist, int nblocks,
+                                                   uint32_t* __restrict__ dbase) {
+  __shared__ uint32_t seg[4][256];
+  const int d = threadIdx.x & 255, q = threadIdx.x >> 8;
+  const int per = (nblocks + 3) / 4;
+  const int b0 = q * per, b1 = min(nblocks, b0 + per);
+  uint32_t s = 0;
+  for (int b = b0; b < b1; ++b) s += hist[(int64_t)b * 256 + d];
+  seg[q][d] = s;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[threadIdx.x] - sum;
-  for (int64_t i = s; i < e; ++i) {
-    const uint32_t v = a[i];
-    a[i] = run;
+  uint32_t run = 0;
+  for (int k = 0; k < q; ++k) run += seg[k][d];
+  for (int b = b0; b < b1; ++b) {
+    const uint32_t v = hist[(int64_t)b * 256 + d];
+    hist[(int64_t)b * 256 + d] = run;
     run += v;
+  }
+  // Digit totals -> exclusive prefix: threads 0..255 (q == 0, waves 0-3) scan
+  // their wave with shuffles; wave sums combine through LDS. Every thread
+  // reaches both barriers.
+  __shared__ uint32_t wsum[4];
+  const uint32_t mine = seg[0][d] + seg[1][d] + seg[2][d] + seg[3][d];
+  uint32_t incl = mine;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (q == 0) {
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+  }
+  __syncthreads();
+  if (q == 0) {
+    uint32_t add = 0;
+    for (int w = 0; w < wave; ++w) add += wsum[w];
+    dbase[d] = add + incl - mine;
   }
 }
 
 __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
-    int nblocks, const uint32_t* __restrict__ hist, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout) {
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ dbase,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
   __shared__ uint32_t base[256];
   __shared__ uint32_t wcount[4][256];
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  base[t] = hist[(int64_t)t * nblocks + blockIdx.x];
+  base[t] = hist[(int64_t)blockIdx.x * 256 + t] + dbase[t];
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   const int64_t tile = (int64_t)blockIdx.x * kSortTile;
   for (int r = 0; r < kSortItems; ++r) {
@@ -109,7 +129,7 @@ int SortWorkspace::ensure(fx_ctx* ctx, int64_t n) {
   const int64_t nb = (n + kSortTile - 1) / kSortTile;
   int rc;
   if ((rc = keys_alt.alloc(ctx, sizeof(uint64_t) * n)) || (rc = vals_alt.alloc(ctx, sizeof(uint32_t) * n)) ||
-      (rc = hist.alloc(ctx, sizeof(uint32_t) * 256 * nb)))
+      (rc = hist.alloc(ctx, sizeof(uint32_t) * 256 * (nb + 1))))
     return rc;
   capacity = n;
   blocks = nb;
@@ -127,12 +147,13 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
   uint64_t* kb = ws.keys_alt.as<uint64_t>();
   uint32_t* vb = ws.vals_alt.as<uint32_t>();
   uint32_t* hist = ws.hist.as<uint32_t>();
+  uint32_t* dbase = hist + (size_t)256 * nb;
   int passes = 0;
   for (int shift = lo_bit; shift < hi_bit; shift += 8, ++passes) {
-    hipLaunchKernelGGL(radix_upsweep, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, nb, hist);
-    hipLaunchKernelGGL(scan_single_block_u32, dim3(1), dim3(1024), 0, st, hist, (int64_t)256 * nb);
-    hipLaunchKernelGGL(radix_downsweep, dim3(nb), dim3(kSortThreads), 0, st, ka, va, n, shift, nb,
-                       hist, kb, vb);
+    hipLaunchKernelGGL(radix_upsweep, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, hist);
+    hipLaunchKernelGGL(radix_scan, dim3(1), dim3(1024), 0, st, hist, nb, dbase);
+    hipLaunchKernelGGL(radix_downsweep, dim3(nb), dim3(kSortThreads), 0, st, ka, va, n, shift,
+                       (const uint32_t*)hist, (const uint32_t*)dbase, kb, vb);
     FX_HIP(hipGetLastError());
     std::swap(ka, kb);
     std::swap(va, vb);

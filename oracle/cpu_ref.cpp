@@ -420,7 +420,9 @@ ORC_API void orc_springs_f64(int32_t n, int32_t e, const int32_t* net_ptr,
 //     force walk skipping the leaf that holds p_idx (149), accept leaf or
 //     size/dist < theta, F = k q Q / d^2 along (p-c)/d with d >= min_dist.
 // (2) bh_record: the FP32 path of record. Same insertion tree and walk, with
-//     three stated changes: the cell geometry stays in double (FP32
+//     four stated changes (the fourth: a particle's force is summed per root
+//     quadrant and the four partial sums are added in quadrant order, so
+//     that the quadrants can be walked in parallel): the cell geometry stays in double (FP32
 //     midpoints stop halving below one ulp), the co-location merge happens at
 //     depth D = min(32, first depth whose cell size < 1e-8) (the same rule
 //     whenever that depth is <= 32), and a node's centre is sum(q x)/sum(q)
@@ -672,32 +674,50 @@ ORC_API void orc_bh_record_f32(int32_t n, const float* x, const float* y, float 
 #endif
   for (int32_t p = 0; p < n; ++p) {
     const float px = x[p], py = y[p], charge = 1.0f;
-    float ax = 0.0f, ay = 0.0f;
-    int stack[4 * 64 + 8];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp) {
-      const RecNode& nd = nodes[stack[--sp]];
-      if (nd.count == 0) continue;
-      const bool lf = nd.leaf();
-      if (lf && nd.first == p) continue;
-      const float dx = px - nd.cx, dy = py - nd.cy;
-      const float dist_sq = dx * dx + dy * dy;
-      float dist = std::sqrt(dist_sq);
-      if (dist < min_dist) dist = min_dist;
-      if (lf || (nd.size / dist < theta)) {
-        const float f = (k * charge * nd.q) / (dist * dist);
-        ax += f * (dx / dist);
-        ay += f * (dy / dist);
-        ++inter;
-      } else {
-        ++opened;
-        for (int c = 3; c >= 0; --c)
-          if (nd.child[c] >= 0) stack[sp++] = nd.child[c];
+    // One walk of computeForceRec (BarnesHutTree.cpp:145-168) in DFS order,
+    // summed per root quadrant: part[r] collects the r-th existing child's
+    // subtree (a root accepted as a whole goes to part[0]); the total is
+    // ((part0 + part1) + part2) + part3.
+    float pax[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pay[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    auto visit = [&](int start, bool split_root, float& ax, float& ay) -> bool {
+      int stack[4 * 64 + 8];
+      int sp = 0;
+      stack[sp++] = start;
+      while (sp) {
+        const RecNode& nd = nodes[stack[--sp]];
+        if (nd.count == 0) continue;
+        const bool lf = nd.leaf();
+        if (lf && nd.first == p) continue;
+        const float dx = px - nd.cx, dy = py - nd.cy;
+        const float dist_sq = dx * dx + dy * dy;
+        float dist = std::sqrt(dist_sq);
+        if (dist < min_dist) dist = min_dist;
+        if (lf || (nd.size / dist < theta)) {
+          const float f = (k * charge * nd.q) / (dist * dist);
+          ax += f * (dx / dist);
+          ay += f * (dy / dist);
+          ++inter;
+        } else if (split_root) {
+          return false;  // the root itself is opened: split over its children
+        } else {
+          ++opened;
+          for (int c = 3; c >= 0; --c)
+            if (nd.child[c] >= 0) stack[sp++] = nd.child[c];
+        }
       }
+      return true;
+    };
+    if (!visit(0, true, pax[0], pay[0])) {
+      ++opened;
+      int r = 0;
+      for (int c = 0; c < 4; ++c)
+        if (nodes[0].child[c] >= 0) {
+          visit(nodes[0].child[c], false, pax[r], pay[r]);
+          ++r;
+        }
     }
-    fx[p] = ax;
-    fy[p] = ay;
+    fx[p] = ((pax[0] + pax[1]) + pax[2]) + pax[3];
+    fy[p] = ((pay[0] + pay[1]) + pay[2]) + pay[3];
   }
   if (stats) {
     stats[0] = inter;
