@@ -61,6 +61,71 @@ def phase_roofline(phase, N, E, P, G, stats, avg_ms):
             "frac": ach / PEAK_HBM_GBS, "algorithmic": f"{bytes_per:.4g} B"}
 
 
+def grid_for(n):
+    return 128 if n < 20000 else 256 if n < 100000 else 512 if n < 300000 else 1024
+
+
+def serving(fxn, ctx, num_requests, iters, seed_rng=7):
+    """Placement serving stream: log-uniform 1k-500k gates, grid 128-1024 by
+    size. Plain = one request at a time, eager launches; served = the
+    continuous-batching server (burst for capacity, then Poisson arrivals at
+    0.7x capacity for latency)."""
+    import numpy as np
+    rng = np.random.default_rng(seed_rng)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(500000), num_requests)).astype(int)
+    reqs = []
+    for r, n in enumerate(sizes):
+        circ, x, y = fxn.netgen.flat(int(n), int(n) * 11 // 10, fxn.netgen.SERVE_SEED_BASE + r)
+        G = grid_for(n)
+        reqs.append((circ, fxn.PlacerConfig(density_grid=G, density_weight=fxn.netgen.density_weight_for(G)),
+                     x, y))
+    out = {"requests": num_requests, "iterations_per_request": iters,
+           "gates_min": int(sizes.min()), "gates_max": int(sizes.max()),
+           "gates_mean": float(sizes.mean())}
+    t0 = time.perf_counter()
+    for circ, cfg, x, y in reqs:
+        job = fxn.FlatJob(ctx, circ, cfg)
+        job.set_positions(x, y)
+        job.run(iters)
+        job.hpwl()
+        job.close()
+    plain = time.perf_counter() - t0
+    out["plain_placements_per_s"] = num_requests / plain
+    srv = fxn.PlacementServer(ctx, max_active=16, hbm_budget=64 << 30)
+    t0 = time.perf_counter()
+    ids = [srv.submit(c, cfg, x, y, iters) for c, cfg, x, y in reqs]
+    srv.drain()
+    burst = time.perf_counter() - t0
+    for i in ids:
+        srv.release(i)
+    cap = num_requests / burst
+    out["served_burst_placements_per_s"] = cap
+    rate = 0.7 * cap
+    gaps = rng.exponential(1.0 / rate, num_requests)
+    c0 = srv.clock()
+    ids = []
+    t_next = time.perf_counter()
+    for (c, cfg, x, y), gap in zip(reqs, gaps):
+        t_next += gap
+        while time.perf_counter() < t_next:
+            time.sleep(min(0.001, max(0.0, t_next - time.perf_counter())))
+        ids.append(srv.submit(c, cfg, x, y, iters))
+    srv.drain()
+    lat, fin = [], []
+    for i in ids:
+        r = srv.poll(i, False)
+        lat.append(r["t_finish"] - r["t_submit"])
+        fin.append(r["t_finish"])
+    srv.close()
+    span = max(fin) - c0
+    out.update({"poisson_rate_per_s": rate, "served_placements_per_s": num_requests / span,
+                "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+                "served_over_plain": (num_requests / span) / out["plain_placements_per_s"],
+                "burst_over_plain": cap / out["plain_placements_per_s"]})
+    return out
+
+
 def cpu_baseline(circ, x, y, G, dw, seconds):
     """FP32 CPU path of record (the oracle) on a bounded sample of iterations."""
     from oracle import oracle as orc
@@ -89,8 +154,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="flat_200k", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--density-weight", type=float, default=1e-4)
+    ap.add_argument("--density-weight", type=float, default=None,
+                    help="default: fluxion.netgen.density_weight_for(grid)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests in the serving measurement (0 skips it)")
+    ap.add_argument("--serve-iters", type=int, default=100)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (HBM counter calibration)")
+    ap.add_argument("--fast-math", action="store_true",
+                    help="Barnes-Hut walk with rsq/FMA instead of IEEE div/sqrt")
     ap.add_argument("--graph", action="store_true",
                     help="replay a captured hipGraph per step (no per-kernel events)")
     args = ap.parse_args()
@@ -98,6 +171,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
+        return
 
     import numpy as np
 
@@ -108,8 +191,11 @@ def main():
     W = args.warmup
     circ, x0, y0 = fxn.netgen.flat(N, E, fxn.netgen.SEED_BASE + seed_off)
     P = int(circ.net_ptr[-1])
+    if args.density_weight is None:
+        args.density_weight = fxn.netgen.density_weight_for(G)
     ctx = fxn.GpuContext.new(0)
-    cfg = fxn.PlacerConfig(density_grid=G, density_weight=args.density_weight)
+    cfg = fxn.PlacerConfig(density_grid=G, density_weight=args.density_weight,
+                           fast_math=args.fast_math)
     job = fxn.FlatJob(ctx, circ, cfg)
     job.set_positions(x0, y0)
 
@@ -160,7 +246,8 @@ def main():
                    "density_grid": G, "density_weight": args.density_weight,
                    "congestion_grid": C, "theta": 0.5,
                    "dct": "split-bf16 MFMA (3-part split, 6 passes), fp32 accumulate",
-                   "hpwl": "fp64", "execution": "graph" if args.graph else "eager+events"},
+                   "hpwl": "fp64", "execution": "graph" if args.graph else "eager+events",
+                   "bh_walk": "fast (rsq/fma)" if args.fast_math else "exact (ieee div/sqrt)"},
         "final_hpwl": hpwl,
         "routing_overflow": {"total": tot_of, "max": max_of},
         "congestion_cells_nonzero": int((counts > 0).sum()),
@@ -174,6 +261,8 @@ def main():
         rf["avg_launch_ms"] = avg
         rf["traffic"] = None
         out["roofline"] = rf
+    if args.serve_requests > 0:
+        out["serving"] = serving(fxn, ctx, args.serve_requests, args.serve_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(circ, x0, y0, G, args.density_weight, args.cpu_seconds)
         out["cpu_baseline"]["ratio_gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -59,6 +59,10 @@ typedef struct fx_params {
 #define FX_FLAG_NO_SPRINGS 1
 #define FX_FLAG_NO_REPULSION 2
 #define FX_FLAG_NO_DENSITY 4
+/* Barnes-Hut walk with v_rsq_f32 + FMA and a squared opening test instead of
+ * the reference's IEEE division/sqrt sequence: forces within 1e-6 relative of
+ * the exact walk (which is bit-identical to the FP32 CPU path of record). */
+#define FX_FLAG_FAST_MATH 8
 
 /* Fills PlacerConfig::default() and the flat-loop defaults
  * (theta 0.5, min_dist 0.1, density weight 1.0, target 0.8, grid 0). */

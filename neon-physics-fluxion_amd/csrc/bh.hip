@@ -311,16 +311,16 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
     const int count = end - i + 1;
     const long long sx = fixpre[end + 1].a - fixpre[i].a;
     const long long sy = fixpre[end + 1].b - fixpre[i].b;
-    BhNode nd;
-    nd.cx = (float)(r0 + ((double)sx / count) / scale);
-    nd.cy = (float)(r1 + ((double)sy / count) / scale);
-    nd.size = (float)fmax(xh - xl, yh - yl);
-    nd.q = (float)count;
-    nd.skip = (end + 1 < n) ? offset[end + 1] : M;
-    nd.leaf = is_leaf ? (int)vals[i] : -1;
-    nd.count = count;
-    nd.depth = d;
-    nodes[id] = nd;
+    float4 nd;
+    nd.x = (float)(r0 + ((double)sx / count) / scale);
+    nd.y = (float)(r1 + ((double)sy / count) / scale);
+    nd.z = is_leaf ? __int_as_float((int)vals[i]) : (float)fmax(xh - xl, yh - yl);
+    nd.w = is_leaf ? -(float)count : (float)count;
+    BhNode rec;
+    rec.a = nd;
+    rec.skip = (end + 1 < n) ? offset[end + 1] : M;
+    rec.pad0 = rec.pad1 = rec.pad2 = 0;
+    nodes[id] = rec;
   }
 }
 
@@ -334,12 +334,43 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
 // quadrant; the four are added in quadrant order, as the path of record
 // does, so every W gives the same bits. More lanes per particle put more
 // waves in flight to hide the latency of the dependent node loads.
-template <bool kStats, int W>
+// Per-node interaction of the walk. Exact: the reference's operations with
+// IEEE division and square root (bit-identical to the FP32 path of record).
+// Fast: the same law with v_rsq_f32 and FMA and the opening test squared
+// (size^2 < theta^2 max(d^2, min_dist^2)); forces within 1e-6 relative.
+template <bool kFast>
+__device__ __forceinline__ bool bh_node(const float4& nd, bool leaf, float px, float py, float k,
+                                        float theta, float min_dist, float& ax, float& ay) {
+  const float dx = px - nd.x, dy = py - nd.y;
+  const float q = leaf ? -nd.w : nd.w;
+  if (kFast) {
+    const float d2 = fmaxf(fmaf(dx, dx, dy * dy), min_dist * min_dist);
+    if (!leaf && !(nd.z * nd.z < theta * theta * d2)) return false;
+    const float r = __builtin_amdgcn_rsqf(d2);
+    const float f = k * q * (r * r * r);
+    ax = fmaf(f, dx, ax);
+    ay = fmaf(f, dy, ay);
+    return true;
+  }
+  const float dist_sq = dx * dx + dy * dy;
+  float dist = sqrtf(dist_sq);
+  if (dist < min_dist) dist = min_dist;
+  if (leaf || (nd.z / dist < theta)) {
+    const float f = (k * 1.0f * q) / (dist * dist);
+    ax += f * (dx / dist);
+    ay += f * (dy / dist);
+    return true;
+  }
+  return false;
+}
+
+template <bool kStats, int W, bool kFast>
 __global__ __launch_bounds__(kBlock) void bh_traverse(
     int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
     const float* __restrict__ y, const BhGeometry* __restrict__ g,
-    const BhNode* __restrict__ nodes, float k, float theta, float min_dist,
-    float* __restrict__ fx, float* __restrict__ fy, unsigned long long* __restrict__ stats) {
+    const BhNode* __restrict__ nodes, float k, float theta,
+    float min_dist, float* __restrict__ fx, float* __restrict__ fy,
+    unsigned long long* __restrict__ stats) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = gid / W, r = gid % W;
   const bool valid = t < n;
@@ -351,23 +382,21 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
     }
     return;
   }
-  const float charge = 1.0f;
   float qx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, qy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   unsigned long long inter = 0, opened = 0;
   if (valid) {
     const float px = x[p], py = y[p];
-    const BhNode root = nodes[0];
+    const BhNode rn = nodes[0];
+    const float4 root = rn.a;
+    const bool root_leaf = root.w < 0.0f;
+    const int root_skip = rn.skip;
     bool split = false;
-    if (root.leaf != p) {
-      const float dx = px - root.cx, dy = py - root.cy;
-      const float dist_sq = dx * dx + dy * dy;
-      float dist = sqrtf(dist_sq);
-      if (dist < min_dist) dist = min_dist;
-      if (root.leaf >= 0 || (root.size / dist < theta)) {
+    if (!(root_leaf && __float_as_int(root.z) == p)) {
+      float rx = 0.0f, ry = 0.0f;
+      if (bh_node<kFast>(root, root_leaf, px, py, k, theta, min_dist, rx, ry)) {
         if (r == 0) {
-          const float f = (k * charge * root.q) / (dist * dist);
-          qx[0] += f * (dx / dist);
-          qy[0] += f * (dy / dist);
+          qx[0] = rx;
+          qy[0] = ry;
           if (kStats) ++inter;
         }
       } else {
@@ -377,40 +406,38 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
     }
     if (split) {
       int c = 1;
-      for (int j = 0; j < r && c < root.skip; ++j) c = nodes[c].skip;
+      for (int j = 0; j < r && c < root_skip; ++j) c = nodes[c].skip;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (q % W != r) continue;
-        if (c >= root.skip) break;
+        if (c >= root_skip) break;
         int cur = c;
         const int end = nodes[c].skip;
         float ax = 0.0f, ay = 0.0f;
         while (cur < end) {
-          const BhNode nd = nodes[cur];
-          if (nd.skip <= cur) break;  // malformed tree: never loop
-          if (nd.leaf == p) {
-            cur = nd.skip;
-            continue;
+          const BhNode rec = nodes[cur];
+          const float4 nd = rec.a;
+          const bool leaf = nd.w < 0.0f;
+          bool move_on = leaf && __float_as_int(nd.z) == p;
+          if (!move_on) {
+            move_on = bh_node<kFast>(nd, leaf, px, py, k, theta, min_dist, ax, ay);
+            if (kStats) {
+              if (move_on) ++inter;
+              else ++opened;
+            }
           }
-          const float dx = px - nd.cx, dy = py - nd.cy;
-          const float dist_sq = dx * dx + dy * dy;
-          float dist = sqrtf(dist_sq);
-          if (dist < min_dist) dist = min_dist;
-          if (nd.leaf >= 0 || (nd.size / dist < theta)) {
-            const float f = (k * charge * nd.q) / (dist * dist);
-            ax += f * (dx / dist);
-            ay += f * (dy / dist);
-            cur = nd.skip;
-            if (kStats) ++inter;
+          if (move_on) {
+            const int nxt = rec.skip;
+            if (nxt <= cur) break;  // malformed tree: never loop
+            cur = nxt;
           } else {
             cur = cur + 1;
-            if (kStats) ++opened;
           }
         }
         qx[q] = ax;
         qy[q] = ay;
         // advance to this lane's next quadrant (W siblings further)
-        for (int j = 0; j < W && c < root.skip; ++j) c = nodes[c].skip;
+        for (int j = 0; j < W && c < root_skip; ++j) c = nodes[c].skip;
       }
     }
   }
@@ -437,15 +464,23 @@ int launch_walk(Job& j, unsigned long long* stats) {
   BhState& b = j.bh;
   const int W = b.walk_lanes;
   const dim3 grid(div_up((int64_t)W * j.N, kBlock)), block(kBlock);
-#define FX_WALK(WW)                                                                              \
-  hipLaunchKernelGGL((bh_traverse<kStats, WW>), grid, block, 0, j.stream, j.N,                   \
-                     b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),                    \
-                     (const BhGeometry*)b.geom.p, b.nodes.as<BhNode>(), j.p.repel_weight,        \
-                     j.p.theta, j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(), stats)
-  if (W == 1) FX_WALK(1);
-  else if (W == 2) FX_WALK(2);
-  else FX_WALK(4);
+  const bool fast = (j.p.flags & FX_FLAG_FAST_MATH) != 0;
+#define FX_WALK_ARGS                                                                      \
+  grid, block, 0, j.stream, j.N, b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(), \
+      (const BhGeometry*)b.geom.p, b.nodes.as<BhNode>(),                                  \
+      j.p.repel_weight, j.p.theta, j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(), stats
+#define FX_WALK(WW)                                                                      \
+  if (fast) hipLaunchKernelGGL((bh_traverse<kStats, WW, true>), FX_WALK_ARGS);           \
+  else hipLaunchKernelGGL((bh_traverse<kStats, WW, false>), FX_WALK_ARGS)
+  if (W == 1) {
+    FX_WALK(1);
+  } else if (W == 2) {
+    FX_WALK(2);
+  } else {
+    FX_WALK(4);
+  }
 #undef FX_WALK
+#undef FX_WALK_ARGS
   FX_HIP(hipGetLastError());
   return FX_OK;
 }

@@ -519,6 +519,10 @@ static int check_regrow(Job& j, bool* rerun) {
   BhGeometry g;
   FX_HIP(hipMemcpyAsync(&g, j.bh.geom.p, sizeof(g), hipMemcpyDeviceToHost, j.stream));
   FX_HIP(hipStreamSynchronize(j.stream));
+  bool faulted = false;
+  int rc0 = radix_fault(j.bh.sort, j.stream, &faulted);
+  if (rc0) return rc0;
+  if (faulted) return fail(FX_E_DEVICE, "Barnes-Hut key sort: look-back timed out");
   if (!g.overflow) return FX_OK;
   int rc = bh_grow(j, (int64_t)g.num_nodes);
   if (rc) return rc;

@@ -71,6 +71,8 @@ struct SortWorkspace {
 // in place (ping-pong through the workspace). Enqueued on `stream`.
 int radix_sort_pairs(fx_ctx* ctx, hipStream_t stream, SortWorkspace& ws, uint64_t* keys,
                      uint32_t* vals, int64_t n, int lo_bit, int hi_bit);
+// Whether a look-back wait of any sort on this workspace timed out (host sync).
+int radix_fault(SortWorkspace& ws, hipStream_t stream, bool* faulted);
 
 // ---- scans (prims.hip) ----
 // out[0..n] = exclusive scan of in[0..n), total at out[n]. tmp holds

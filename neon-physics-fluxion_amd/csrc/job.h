@@ -17,15 +17,13 @@ struct BhGeometry {
   int32_t overflow;   // set when M exceeded the node capacity
 };
 
-// One quadtree node in DFS pre-order (32 bytes).
-struct __align__(16) BhNode {
-  float cx, cy;     // charge-weighted centre
-  float size;       // cell size max(w, h)
-  float q;          // total charge (particle count: unit charges)
-  int32_t skip;     // pre-order index of the next node after this subtree
-  int32_t leaf;     // first particle of a leaf (lowest index), -1 if internal
-  int32_t count;
-  int32_t depth;
+// Quadtree node in DFS pre-order, one 32-byte record (a single round trip
+// per visit of the walk): (cx, cy, s, q) — internal cell: s = cell size,
+// q = +count; leaf: s = the bits of its first particle's index, q = -count —
+// and the skip pointer (first node after the subtree).
+struct __align__(32) BhNode {
+  float4 a;
+  int32_t skip, pad0, pad1, pad2;
 };
 
 struct BhState {

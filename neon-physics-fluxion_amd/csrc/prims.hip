@@ -1,12 +1,20 @@
 // Device-wide primitives: stable LSD radix sort of (u64 key, u32 value)
 // pairs and exclusive prefix scans (int32, int64 pairs).
 //
-// Radix sort: 8-bit digits. Per pass, an upsweep kernel counts each tile's
-// digits into a digit-major histogram, one workgroup scans it, and a
-// downsweep kernel ranks the tile's items stably — items are processed in
-// rounds of 256 in input order; inside a 64-lane wave, eight ballots match
-// lanes with equal digits (rank = popcount of lower matching lanes), and
-// per-wave digit counts combine through LDS — then scatters them.
+// Radix sort (8-bit digits, "onesweep" structure): one kernel counts the
+// digits of all passes at once and one small kernel turns them into global
+// digit bases; then each pass is ONE kernel over tiles of 1024 items. A tile
+// takes the next tile id from an atomic counter (ids follow start order, so a
+// tile only ever waits on tiles that are already running), counts its
+// digits, publishes them, and sums its predecessors' counts per digit by
+// decoupled look-back: a predecessor's status word holds either its own
+// count (aggregate) or its inclusive prefix (which ends the walk back). The
+// tile then ranks its items stably — rounds of 256 in input order; inside a
+// 64-lane wave, eight ballots match lanes with equal digits — and scatters
+// them. Status words of the next pass are cleared by the current one
+// (double buffering), so nothing needs a memset inside a captured graph.
+// The look-back spin is bounded: on timeout a flag is raised and the result
+// is marked invalid instead of the GPU hanging.
 
 #include "internal.h"
 
@@ -15,86 +23,116 @@ namespace fx {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr uint32_t kStAgg = 1u << 30, kStInc = 2u << 30, kStMask = (1u << 30) - 1;
+constexpr int kSpinLimit = 1 << 22;
 
-__global__ __launch_bounds__(kSortThreads) void radix_upsweep(const uint64_t* __restrict__ keys,
-                                                              int64_t n, int shift,
-                                                              uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
-#pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = base + r * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFF], 1u);
-  }
-  __syncthreads();
-  hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+// Zeroes the 8x256 global digit counts and the per-pass tile counters.
+__global__ void radix_reset(uint32_t* __restrict__ ghist, uint32_t* __restrict__ counters) {
+  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) ghist[i] = 0;
+  if (threadIdx.x < 8) counters[threadIdx.x] = 0;
 }
 
-// Exclusive prefix over the tiles of every digit, in place, then the
-// exclusive prefix of the digit totals into dbase. 1024 threads: digit d is
-// scanned by four threads, each over a quarter of the tiles (loads coalesced
-// across digits), joined through LDS.
-__global__ __launch_bounds__(1024) void radix_scan(uint32_t* __restrict__ hist, int nblocks,
-                                                   uint32_t* __restrict__ dbase) {
-  __shared__ uint32_t seg[4][256];
-  const int d = threadIdx.x & 255, q = threadIdx.x >> 8;
-  const int per = (nblocks + 3) / 4;
-  const int b0 = q * per, b1 = min(nblocks, b0 + per);
-  uint32_t s = 0;
-  for (int b = b0; b < b1; ++b) s += hist[(int64_t)b * 256 + d];
-  seg[q][d] = s;
+// Digit counts of every pass in one read of the keys.
+__global__ __launch_bounds__(256) void radix_histogram(const uint64_t* __restrict__ keys,
+                                                       int64_t n, int lo_bit, int passes,
+                                                       uint32_t* __restrict__ ghist) {
+  __shared__ uint32_t h[8][256];
+  for (int p = 0; p < 8; ++p) h[p][threadIdx.x] = 0;
   __syncthreads();
-  uint32_t run = 0;
-  for (int k = 0; k < q; ++k) run += seg[k][d];
-  for (int b = b0; b < b1; ++b) {
-    const uint32_t v = hist[(int64_t)b * 256 + d];
-    hist[(int64_t)b * 256 + d] = run;
-    run += v;
-  }
-  // Digit totals -> exclusive prefix: threads 0..255 (q == 0, waves 0-3) scan
-  // their wave with shuffles; wave sums combine through LDS. Every thread
-  // reaches both barriers.
-  __shared__ uint32_t wsum[4];
-  const uint32_t mine = seg[0][d] + seg[1][d] + seg[2][d] + seg[3][d];
-  uint32_t incl = mine;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (q == 0) {
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
-    }
-    if (lane == 63) wsum[wave] = incl;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (lo_bit + 8 * p)) & 0xFF], 1u);
   }
   __syncthreads();
-  if (q == 0) {
-    uint32_t add = 0;
-    for (int w = 0; w < wave; ++w) add += wsum[w];
-    dbase[d] = add + incl - mine;
+  for (int p = 0; p < passes; ++p)
+    if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], h[p][threadIdx.x]);
+}
+
+// Exclusive prefix of each pass's 256 digit counts (one wave per pass).
+__global__ void radix_bases(uint32_t* __restrict__ ghist, int passes) {
+  const int p = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (p >= passes) return;
+  uint32_t v[4], s = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = ghist[p * 256 + lane * 4 + k];
+    s += v[k];
+  }
+  uint32_t incl = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += u;
+  }
+  uint32_t run = incl - s;
+  for (int k = 0; k < 4; ++k) {
+    ghist[p * 256 + lane * 4 + k] = run;
+    run += v[k];
   }
 }
 
-__global__ __launch_bounds__(kSortThreads) void radix_downsweep(
+__global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ dbase,
-    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    int ntiles, const uint32_t* __restrict__ gbase, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ status_next, uint32_t* __restrict__ counter,
+    uint32_t* __restrict__ fault, uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+  __shared__ uint32_t cnt[256];
   __shared__ uint32_t base[256];
   __shared__ uint32_t wcount[4][256];
+  __shared__ int tile_s;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  base[t] = hist[(int64_t)blockIdx.x * 256 + t] + dbase[t];
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const int64_t tile = (int64_t)blockIdx.x * kSortTile;
+  if (t == 0) tile_s = (int)atomicAdd(counter, 1u);
+  cnt[t] = 0;
+  __syncthreads();
+  const int tile = tile_s;
+  status_next[(int64_t)tile * 256 + t] = 0;
+  uint64_t k[kSortItems];
+  uint32_t v[kSortItems];
+  const int64_t t0 = (int64_t)tile * kSortTile;
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = tile + r * kSortThreads + t;
-    const bool valid = i < n;
-    uint64_t k = 0;
-    uint32_t v = 0;
-    if (valid) {
-      k = kin[i];
-      v = vin[i];
+    const int64_t i = t0 + r * kSortThreads + t;
+    k[r] = 0;
+    v[r] = 0;
+    if (i < n) {
+      k[r] = kin[i];
+      v[r] = vin[i];
+      atomicAdd(&cnt[(uint32_t)(k[r] >> shift) & 0xFFu], 1u);
     }
-    const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
+  }
+  __syncthreads();
+  // Publish this tile's counts, then look back for the exclusive prefix.
+  const uint32_t mine = cnt[t];
+  uint32_t* st = status + (int64_t)tile * 256 + t;
+  if (tile == 0) {
+    __hip_atomic_store(st, kStInc | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    base[t] = gbase[t];
+  } else {
+    __hip_atomic_store(st, kStAgg | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t excl = 0;
+    int b = tile - 1, spins = 0;
+    while (b >= 0) {
+      const uint32_t w = __hip_atomic_load(status + (int64_t)b * 256 + t, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      if ((w & ~kStMask) == 0) {
+        if (++spins > kSpinLimit) {
+          atomicOr(fault, 1u);
+          break;
+        }
+        continue;
+      }
+      excl += w & kStMask;
+      if (w & kStInc) break;
+      --b;
+    }
+    __hip_atomic_store(st, kStInc | (excl + mine), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    base[t] = gbase[t] + excl;
+  }
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = t0 + r * kSortThreads + t;
+    const bool valid = i < n;
+    const uint32_t d = (uint32_t)(k[r] >> shift) & 0xFFu;
     wcount[0][t] = 0;
     wcount[1][t] = 0;
     wcount[2][t] = 0;
@@ -102,9 +140,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     __syncthreads();
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t m = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? m : ~m;
+    for (int bit = 0; bit < 8; ++bit) {
+      const uint64_t m = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? m : ~m;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
     if (valid && rank == 0) wcount[wave][d] = (uint32_t)__popcll(peers);
@@ -114,25 +152,26 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
       for (int w = 0; w < wave; ++w) off += wcount[w][d];
     const uint32_t pos = base[d] + off + rank;
     __syncthreads();
-    // Advance per-digit bases by this round's totals (thread t owns digit t).
     base[t] += wcount[0][t] + wcount[1][t] + wcount[2][t] + wcount[3][t];
     if (valid) {
-      kout[pos] = k;
-      vout[pos] = v;
+      kout[pos] = k[r];
+      vout[pos] = v[r];
     }
-    __syncthreads();
   }
 }
 
 int SortWorkspace::ensure(fx_ctx* ctx, int64_t n) {
   if (n <= capacity) return FX_OK;
-  const int64_t nb = (n + kSortTile - 1) / kSortTile;
+  const int64_t nt = (n + kSortTile - 1) / kSortTile;
   int rc;
+  // hist: 8x256 bases | 2 x ntiles x 256 status words | 8 counters | fault flag
+  const size_t words = 8 * 256 + 2 * (size_t)nt * 256 + 8 + 8;
   if ((rc = keys_alt.alloc(ctx, sizeof(uint64_t) * n)) || (rc = vals_alt.alloc(ctx, sizeof(uint32_t) * n)) ||
-      (rc = hist.alloc(ctx, sizeof(uint32_t) * 256 * (nb + 1))))
+      (rc = hist.alloc(ctx, sizeof(uint32_t) * words)))
     return rc;
+  FX_HIP(hipMemset(hist.p, 0, sizeof(uint32_t) * words));
   capacity = n;
-  blocks = nb;
+  blocks = nt;
   return FX_OK;
 }
 
@@ -141,27 +180,53 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
   if (n <= 1) return FX_OK;
   int rc = ws.ensure(ctx, n);
   if (rc) return rc;
-  const int nb = (int)((n + kSortTile - 1) / kSortTile);
+  const int nt = (int)((n + kSortTile - 1) / kSortTile);
+  const int passes = (hi_bit - lo_bit + 7) / 8;
+  if (passes < 1 || passes > 8) return fail(FX_E_INVALID, "radix sort: bad bit range");
+  uint32_t* gbase = ws.hist.as<uint32_t>();
+  uint32_t* status0 = gbase + 8 * 256;
+  uint32_t* status1 = status0 + (size_t)ws.blocks * 256;
+  uint32_t* counters = status1 + (size_t)ws.blocks * 256;
+  uint32_t* fault = counters + 8;
   uint64_t* ka = keys;
   uint32_t* va = vals;
   uint64_t* kb = ws.keys_alt.as<uint64_t>();
   uint32_t* vb = ws.vals_alt.as<uint32_t>();
-  uint32_t* hist = ws.hist.as<uint32_t>();
-  uint32_t* dbase = hist + (size_t)256 * nb;
-  int passes = 0;
-  for (int shift = lo_bit; shift < hi_bit; shift += 8, ++passes) {
-    hipLaunchKernelGGL(radix_upsweep, dim3(nb), dim3(kSortThreads), 0, st, ka, n, shift, hist);
-    hipLaunchKernelGGL(radix_scan, dim3(1), dim3(1024), 0, st, hist, nb, dbase);
-    hipLaunchKernelGGL(radix_downsweep, dim3(nb), dim3(kSortThreads), 0, st, ka, va, n, shift,
-                       (const uint32_t*)hist, (const uint32_t*)dbase, kb, vb);
-    FX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(radix_reset, dim3(1), dim3(256), 0, st, gbase, counters);
+  const unsigned hblocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(radix_histogram, dim3(hblocks), dim3(256), 0, st, ka, n, lo_bit, passes, gbase);
+  hipLaunchKernelGGL(radix_bases, dim3(1), dim3(512), 0, st, gbase, passes);
+  for (int p = 0; p < passes; ++p) {
+    uint32_t* cur = (p & 1) ? status1 : status0;
+    uint32_t* nxt = (p & 1) ? status0 : status1;
+    hipLaunchKernelGGL(radix_onesweep, dim3(nt), dim3(kSortThreads), 0, st, ka, va, n,
+                       lo_bit + 8 * p, nt, (const uint32_t*)(gbase + p * 256), cur, nxt,
+                       counters + p, fault, kb, vb);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
+  FX_HIP(hipGetLastError());
   if (passes & 1) {
+    // The last pass used the buffer pass 0 reads: clear it for the next sort,
+    // and bring the result back from the alternate buffers.
+    if ((rc = fill_i32(st, reinterpret_cast<int32_t*>(status0), (int64_t)ws.blocks * 256, 0)))
+      return rc;
     FX_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
     FX_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
   }
+  return FX_OK;
+}
+
+int radix_fault(SortWorkspace& ws, hipStream_t st, bool* faulted) {
+  uint32_t f = 0;
+  if (!ws.hist.p) {
+    *faulted = false;
+    return FX_OK;
+  }
+  uint32_t* fault = ws.hist.as<uint32_t>() + 8 * 256 + 2 * (size_t)ws.blocks * 256 + 8;
+  FX_HIP(hipMemcpyAsync(&f, fault, 4, hipMemcpyDeviceToHost, st));
+  FX_HIP(hipStreamSynchronize(st));
+  *faulted = f != 0;
   return FX_OK;
 }
 
@@ -351,6 +416,9 @@ int sort_common(fx_ctx* ctx, int64_t n, const uint64_t* hkeys, const float* hf, 
   FX_HIP(hipMemcpyAsync(perm, v.p, 4 * n, hipMemcpyDeviceToHost, st));
   FX_HIP(hipStreamSynchronize(st));
   FX_CHECK(!nanf, FX_E_INVALID, "fx_sort_f32: NaN key (partial_cmp has no order for NaN)");
+  bool faulted = false;
+  if ((rc = fx::radix_fault(ws, st, &faulted))) return rc;
+  FX_CHECK(!faulted, FX_E_DEVICE, "radix sort look-back timed out");
   return FX_OK;
 }
 

@@ -47,3 +47,16 @@ def leaves(num_leaves: int, seed: int, min_n: int = 16, max_n: int = 256, region
     side = int(np.ceil(np.sqrt(num_leaves))) * region
     circ = Circuit(n, net_ptr, pins[:p], side, side)
     return circ, leaf_ptr, np.arange(n, dtype=np.int32), reg
+
+
+def density_weight_for(G: int) -> float:
+    """Density-force weight of the flat workloads for a DCT grid of side G.
+
+    The reference's eigenvalue (w cos(pi k/n) - w)^2 grows as k^4, so the
+    low modes of the density potential are amplified ~1e10 at G=512 and the
+    force scales ~G^4. Measured on the 200k-gate workload: weight 1.0 sends
+    every gate to a die corner (CPU path too), 1e-4 piles ~90% of the gates
+    against a die edge, 3e-6 keeps all gates interior with the density force
+    ~13% of the springs. The weight is 3e-6 at G=512, scaled by (512/G)^4.
+    """
+    return 3e-6 * (512.0 / G) ** 4 if G else 0.0

@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (FX_FLAG_NO_DENSITY, FX_FLAG_NO_REPULSION, FX_FLAG_NO_SPRINGS, FX_LEAF_EXACT,
+from ._lib import (FX_FLAG_FAST_MATH, FX_FLAG_NO_DENSITY, FX_FLAG_NO_REPULSION, FX_FLAG_NO_SPRINGS, FX_LEAF_EXACT,
                    FX_LEAF_FAST, FluxionError, FxLeafBatch, FxNetlist, FxParams, FxServerConfig,
                    check, default_params, f32p, i32p, lib, ptr)
 
@@ -52,9 +52,10 @@ class PlacerConfig:
     springs: bool = True
     repulsion: bool = True
     density: bool = True
+    fast_math: bool = False  # Barnes-Hut walk with rsq/FMA (FX_FLAG_FAST_MATH)
 
     def to_params(self) -> FxParams:
-        flags = 0
+        flags = FX_FLAG_FAST_MATH if self.fast_math else 0
         if not self.springs:
             flags |= FX_FLAG_NO_SPRINGS
         if not self.repulsion:

@@ -1,6 +1,6 @@
 """Writes flat200k_golden.json: end-of-run metrics of the CPU oracle on the
 200,000-gate flat placement workload (seed 0xF1A00002, 220,000 nets, 1000x1000
-die, theta 0.5, 512x512 DCT grid, density weight 1e-4, 500 iterations from the
+die, theta 0.5, 512x512 DCT grid, density weight 3e-6, 500 iterations from the
 generated positions), in FP64 (the oracle of record for HPWL/overflow) and in
 FP32 (the CPU path of record). Run once; takes ~20 minutes on 8 cores.
 """
@@ -21,7 +21,8 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "flat200k_golden.
 
 
 def main():
-    N, E, G, T, DW, C = 200000, 220000, 512, 500, 1e-4, 256
+    N, E, G, T, C = 200000, 220000, 512, 500, 256
+    DW = fxn.netgen.density_weight_for(G)
     circ, x0, y0 = fxn.netgen.flat(N, E, fxn.netgen.SEED_BASE + 2)
     res = {"workload": "flat_200k", "gates": N, "nets": E, "density_grid": G, "iterations": T,
            "density_weight": DW, "congestion_grid": C}

@@ -20,10 +20,8 @@ def make(N, E, seed, **cfg):
 
 
 fxn_ctx = [None]
-# Density weight of the flat workloads: the reference eigenvalue grows as k^4, so at
-# weight 1.0 the explicit step is unstable (all gates end on a die corner, CPU path
-# included); 1e-4 is the stable setting measured at 200k gates / G=512.
-DW = 1e-4
+# Density weight of the flat workloads (fluxion.netgen.density_weight_for).
+DW = fxn.netgen.density_weight_for(128)
 
 
 @pytest.fixture(autouse=True)
@@ -101,6 +99,19 @@ def test_barnes_hut_vs_record(N):
     assert s["interactions"] == st["interactions"] and s["opened"] == st["opened"]
 
 
+@pytest.mark.parametrize("N", [20000, 200000])
+def test_barnes_hut_fast_math_within_tolerance(N):
+    print(f"case: BH fast N={N}", flush=True)
+    circ, x, y = fxn.netgen.flat(N, N, 43 + N)
+    job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(fast_math=True))
+    job.set_positions(x, y)
+    gx, gy = job.forces(2)
+    rx, ry, st = orc.bh_record(x, y, nthreads=8)
+    err = max(nrm_err(gx, rx), nrm_err(gy, ry))
+    print(f"BH fast N={N}: err {err:.2e}", flush=True)
+    assert err < 1e-4
+
+
 def test_barnes_hut_colocated_and_clamped():
     # many gates on the same point (die corner after clamping) and on the die edge
     print("case: BH co-located", flush=True)
@@ -136,10 +147,11 @@ def test_density_fields_vs_fp64(G):
         assert v < 1e-4, (k, v)
 
 
-def test_full_step_teacher_forced():
-    print("case: flat step teacher forced", flush=True)
+@pytest.mark.parametrize("fast", [False, True])
+def test_full_step_teacher_forced(fast):
+    print(f"case: flat step teacher forced fast={fast}", flush=True)
     N, G = 20000, 128
-    circ, x, y, job = make(N, 22000, 61, density_grid=G, density_weight=DW)
+    circ, x, y, job = make(N, 22000, 61, density_grid=G, density_weight=DW, fast_math=fast)
     ox, oy, ovx, ovy = x, y, np.zeros(N, np.float32), np.zeros(N, np.float32)
     for t in range(4):
         job.set_positions(ox, oy)
@@ -154,13 +166,14 @@ def test_full_step_teacher_forced():
                                             ovx, ovy, iters=25, G=G, dens_weight=DW, nthreads=8)
 
 
-def test_run_hpwl_vs_fp64_and_graph_identical():
+@pytest.mark.parametrize("fast", [False, True])
+def test_run_hpwl_vs_fp64_and_graph_identical(fast):
     # End-to-end: 100 iterations at 20k gates, GPU vs the FP64 oracle run from the
     # same start, HPWL within 0.5%. (The dynamics are chaotic: at 4k gates even the
     # FP32 CPU path of record differs from FP64 by up to 1.4%; at 20k by 0.1-0.2%.)
     print("case: flat run vs fp64", flush=True)
     N, G, T = 20000, 128, 100
-    circ, x, y, job = make(N, 22000, 71, density_grid=G, density_weight=DW)
+    circ, x, y, job = make(N, 22000, 71, density_grid=G, density_weight=DW, fast_math=fast)
     job.run(T)
     gx, gy = job.positions()
     h = job.hpwl()
@@ -173,10 +186,19 @@ def test_run_hpwl_vs_fp64_and_graph_identical():
     c64 = orc.congestion(circ.net_ptr, circ.net_pins, ox.astype(np.float32), oy.astype(np.float32),
                          1000., 1000., 64, 64)
     rt, rm = orc.overflow(c64, 1000.0, 1000.0)
-    print(f"overflow gpu {ot:.1f} fp64-run {rt:.1f} rel {abs(ot - rt) / max(rt, 1):.2e}", flush=True)
-    assert abs(ot - rt) / max(rt, 1.0) < 5e-3
+    # the FP32 CPU path of record's own distance from the FP64 run on this chaotic metric
+    fx32, fy32, *_ = orc.flat_run(N, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, x, y, iters=T,
+                                  G=G, dens_weight=DW, nthreads=8)
+    c32 = orc.congestion(circ.net_ptr, circ.net_pins, fx32, fy32, 1000., 1000., 64, 64)
+    r32, _ = orc.overflow(c32, 1000.0, 1000.0)
+    spread = abs(r32 - rt) / max(rt, 1.0)
+    rel = abs(ot - rt) / max(rt, 1.0)
+    print(f"overflow gpu {ot:.1f} fp64-run {rt:.1f} rel {rel:.2e}; fp32-cpu {r32:.1f} "
+          f"rel {spread:.2e}", flush=True)
+    assert rel < max(5e-3, 2.0 * spread)
     # graph replay == eager, bit for bit
-    job2 = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(density_grid=G, density_weight=DW))
+    job2 = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(density_grid=G, density_weight=DW,
+                                                         fast_math=fast))
     job2.set_positions(x, y)
     job2.use_graph(True)
     for _ in range(4):
@@ -186,7 +208,8 @@ def test_run_hpwl_vs_fp64_and_graph_identical():
     assert np.array_equal(gx, hx) and np.array_equal(gy, hy)
 
 
-def test_full_size_run_vs_fp64_golden():
+@pytest.mark.parametrize("fast", [False, True])
+def test_full_size_run_vs_fp64_golden(fast):
     # 200k gates, 512^2 DCT, 500 iterations vs the FP64 oracle's run (golden file
     # written by tests/golden/make_flat_golden.py): HPWL and overflow within 0.5%.
     import json
@@ -198,7 +221,8 @@ def test_full_size_run_vs_fp64_golden():
     print("case: 200k x 500 vs fp64 golden", flush=True)
     circ, x, y = fxn.netgen.flat(gold["gates"], gold["nets"], fxn.netgen.SEED_BASE + 2)
     job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig(density_grid=gold["density_grid"],
-                                                        density_weight=gold["density_weight"]))
+                                                        density_weight=gold["density_weight"],
+                                                        fast_math=fast))
     job.set_positions(x, y)
     job.use_graph(True)
     job.run(gold["iterations"])
