@@ -36,21 +36,45 @@ WORKLOADS = {
 }
 
 
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel, workload, mode):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter's
+    reading for a plain copy of known size). None when no summary exists."""
+    path = os.path.join(PROFILES, f"{kernel}_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) == 3 and t[0] == "TRAFFIC" and t[1] == mode:
+                return float(t[2])
+    return None
+
+
 def phase_roofline(phase, N, E, P, G, stats, avg_ms):
     """Algorithmic work of one launch of a phase -> roofline dict."""
     s = avg_ms / 1e3
     if phase == "bh_walk":
+        # The walk is FP32 vector-ALU work on an L2-resident tree. MI355X's
+        # published FP32 peak (157.3 TFLOP/s) is the same for the vector and
+        # the matrix pipes; "pipe" names the one this kernel uses.
         flops = 17 * stats["interactions"] + 9 * stats["opened"]
         ach = flops / s / 1e12
-        return {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP32_TFLOPS, "algorithmic": f"17*I+9*V = {flops:.4g} flop"}
+        return {"bound": "mfma", "pipe": "valu-fp32", "achieved": ach, "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
+                "algorithmic": f"17*I+9*V = {flops:.4g} flop (I={stats['interactions']}, "
+                               f"V={stats['opened']})"}
     if phase == "dct":
         flops = 8 * G ** 3
         ach = flops / s / 1e12
-        return {"bound": "mfma", "achieved": ach, "peak": PEAK_MFMA_BF16_TFLOPS, "unit": "TFLOP/s",
+        return {"bound": "mfma", "pipe": "mfma-bf16", "achieved": ach,
+                "peak": PEAK_MFMA_BF16_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_MFMA_BF16_TFLOPS, "algorithmic": f"8*G^3 = {flops:.4g} flop"}
     bytes_per = {
-        "springs": 4 * (E + 1) + 12 * P + 16 * E,
+        "springs": 20 * N + 8 * P + 20 * E,
         "bh_keys": 8 * N + 12 * N,
         "bh_sort": 8 * 36 * N,
         "bh_tree": 150 * N,
@@ -59,6 +83,26 @@ def phase_roofline(phase, N, E, P, G, stats, avg_ms):
     ach = bytes_per / s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS, "algorithmic": f"{bytes_per:.4g} B"}
+
+
+def iteration_roofline(N, E, P, G, C, K, stats, elapsed):
+    """Ideal time of the timed region from algorithmic counts, per kernel
+    max(bytes/HBM, mfma flops/MFMA peak, valu flops/FP32 peak), summed, over
+    the measured time: springs 20N+8P+20E B, integrator 40N B, Barnes-Hut
+    188N B + (17I+9V) flop, DCT 8G^3 flop + 28N+38G^2 B per iteration; once
+    per run congestion 32E+12C^2 B and HPWL 16E B."""
+    hbm, mf, va = PEAK_HBM_GBS * 1e9, PEAK_MFMA_BF16_TFLOPS * 1e12, PEAK_FP32_TFLOPS * 1e12
+    per_iter = {
+        "springs": (20 * N + 8 * P + 20 * E) / hbm,
+        "integrator": 40 * N / hbm,
+        "bh": max(188 * N / hbm, (17 * stats["interactions"] + 9 * stats["opened"]) / va),
+        "dct": max(8 * G ** 3 / mf, (28 * N + 38 * G * G) / hbm) if G else 0.0,
+    }
+    once = (32 * E + 12 * C * C + 16 * E) / hbm
+    ideal = K * sum(per_iter.values()) + once
+    return {"ideal_us_per_iter": ideal / K * 1e6, "measured_us_per_iter": elapsed / K * 1e6,
+            "frac": ideal / elapsed,
+            "ideal_us_terms": {k: v * 1e6 for k, v in per_iter.items()}}
 
 
 def grid_for(n):
@@ -126,7 +170,7 @@ def serving(fxn, ctx, num_requests, iters, seed_rng=7):
     return out
 
 
-def cpu_baseline(circ, x, y, G, dw, seconds):
+def cpu_baseline(circ, x, y, G, dw, seconds, nthreads=1):
     """FP32 CPU path of record (the oracle) on a bounded sample of iterations."""
     from oracle import oracle as orc
     n = circ.num_gates
@@ -136,15 +180,85 @@ def cpu_baseline(circ, x, y, G, dw, seconds):
     while True:
         xx, yy, vx, vy, *_ = orc.flat_run(n, circ.net_ptr, circ.net_pins, 1000.0, 1000.0, xx, yy,
                                           vx, vy, iters=1, G=G, dens_weight=dw,
-                                          nthreads=1)
+                                          nthreads=nthreads)
         its += 1
         el = time.perf_counter() - t0
-        if el >= seconds or its >= 50:
+        if el >= seconds or its >= 200:
             break
-    return {"value": its / el, "unit": "placement iters/s", "cores": 1, "kind": "port",
+    return {"value": its / el, "unit": "placement iters/s", "cores": nthreads, "kind": "port",
             "sample": f"{its} iterations of the FP32 CPU path of record (oracle/cpu_ref.cpp: "
-                      f"springs + Barnes-Hut + DCT G={G} + integrator) at {n} gates, 1 thread, "
-                      f"{el:.1f} s"}
+                      f"springs + Barnes-Hut + DCT G={G} + integrator) at {n} gates from the "
+                      f"generated placement, {nthreads} thread(s), {el:.1f} s"}
+
+
+def leaves_workload(fxn, ctx, num_leaves=4096, iters=300):
+    """Batched leaf placement (placer.rs:66-69 as one launch): 4,096 leaves,
+    n ~ log-U[16,256], T=300, exact O(n^2) repulsion, both walk modes."""
+    import numpy as np
+    circ, leaf_ptr, gates, reg = fxn.netgen.leaves(num_leaves, fxn.netgen.SEED_BASE + 3)
+    n = np.diff(leaf_ptr).astype(np.int64)
+    flops = iters * float((17 * n * (n - 1) // 2 + 14 * n).sum())
+    cfg = fxn.PlacerConfig(force_iterations=iters)
+    out = {"leaves": num_leaves, "gates": int(n.sum()), "iterations": iters,
+           "algorithmic_flop": flops}
+    for mode, fast in (("exact", False), ("fast", True)):
+        pl = fxn.Placement.new(circ.num_gates)
+        fxn.place_leaves(ctx, leaf_ptr, gates, reg, circ, pl, cfg, fast=fast)  # warm-up
+        best, wall = 1e30, 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ms = fxn.place_leaves(ctx, leaf_ptr, gates, reg, circ, pl, cfg, fast=fast)
+            wall = min(wall, (time.perf_counter() - t0) * 1e3)
+            best = min(best, ms)
+        ach = flops / (best / 1e3) / 1e12
+        out[mode] = {"kernel_ms": best, "call_ms": wall, "achieved_tflops": ach,
+                     "frac_fp32_peak": ach / PEAK_FP32_TFLOPS,
+                     "leaf_iters_per_s": num_leaves * iters / (best / 1e3)}
+    return out
+
+
+def flat2m_workload(fxn, ctx, iters=20, fiedler_steps=200):
+    """2M gates / 2.2M nets, DCT G=1024, congestion 1024x1024, plus 200
+    power-iteration steps over all gates (partitioner.rs:106-185)."""
+    import numpy as np
+    N, E, off, G, C, _ = WORKLOADS["flat_2m"]
+    circ, x0, y0 = fxn.netgen.flat(N, E, fxn.netgen.SEED_BASE + off)
+    cfg = fxn.PlacerConfig(density_grid=G, density_weight=fxn.netgen.density_weight_for(G))
+    out = {"gates": N, "nets": E, "pins": int(circ.net_ptr[-1]), "density_grid": G,
+           "congestion_grid": C, "iterations": iters}
+    with fxn.FlatJob(ctx, circ, cfg) as job:
+        job.set_positions(x0, y0)
+        job.run(3)
+        job.congestion(C, C)
+        job.set_timing(fxn.FlatJob.PHASES)
+        job.run(3)
+        out["phases_ms_per_iter"] = {k: v[0] / max(1, v[1]) for k, v in job.read_timing().items()}
+        job.set_timing(())
+        job.set_positions(x0, y0)
+        job.set_velocities(np.zeros(N, np.float32), np.zeros(N, np.float32))
+        job.sync()
+        t0 = time.perf_counter()
+        job.run_async(iters)
+        job.congestion(C, C)
+        job.overflow(C, C)
+        h = job.hpwl()
+        job.sync()
+        el = time.perf_counter() - t0
+        out.update({"iters_per_s": iters / el, "ms_per_iter": el / iters * 1e3, "final_hpwl": h,
+                    "bh": job.bh_stats()})
+    t0 = time.perf_counter()
+    aptr, adj = circ.build_adjacency()
+    out["adjacency_build_s"] = time.perf_counter() - t0
+    out["laplacian_nnz"] = int(adj.shape[0])
+    gates = np.arange(N, dtype=np.int32)
+    fxn.compute_fiedler_vector(ctx, circ, gates, 2)
+    fxn.compute_fiedler_vector(ctx, circ, gates, fiedler_steps)
+    ms = float(fxn.lib.fx_fiedler_last_ms(ctx.handle))
+    spmv_bytes = 4 * (N + 1) + 4 * adj.shape[0] + 12 * N
+    out["fiedler"] = {"steps": fiedler_steps, "kernel_ms": ms, "ms_per_step": ms / fiedler_steps,
+                      "spmv_algorithmic_bytes": spmv_bytes,
+                      "spmv_frac_hbm": spmv_bytes / (ms / fiedler_steps / 1e3) / 1e9 / PEAK_HBM_GBS}
+    return out
 
 
 def main():
@@ -166,6 +280,10 @@ def main():
                     help="Barnes-Hut walk with rsq/FMA instead of IEEE div/sqrt")
     ap.add_argument("--graph", action="store_true",
                     help="replay a captured hipGraph per step (no per-kernel events)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-leaf, 2M-gate and graph-replay measurements")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="threads of the all-cores CPU baseline (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -259,13 +377,40 @@ def main():
         rf = phase_roofline(dominant, N, E, P, G, stats, avg)
         rf["kernel"] = dominant
         rf["avg_launch_ms"] = avg
-        rf["traffic"] = None
+        rf["traffic"] = measured_traffic(dominant, args.workload,
+                                         "fast" if args.fast_math else "exact")
         out["roofline"] = rf
+    out["iteration_roofline"] = iteration_roofline(N, E, P, G, C, K, stats, elapsed)
+    if not args.no_extras:
+        if not args.graph:
+            # Same K iterations replayed from one captured hipGraph per step.
+            job.use_graph(True)
+            job.set_timing(())
+            job.set_positions(x0, y0)
+            job.set_velocities(np.zeros(N, np.float32), np.zeros(N, np.float32))
+            job.run(1)
+            job.set_positions(x0, y0)
+            job.set_velocities(np.zeros(N, np.float32), np.zeros(N, np.float32))
+            job.sync()
+            t0 = time.perf_counter()
+            job.run_async(K)
+            hg = job.hpwl()
+            job.sync()
+            out["graph_replay"] = {"iters_per_s": K / (time.perf_counter() - t0),
+                                   "final_hpwl": hg, "same_hpwl_as_eager": hg == hpwl}
+            job.use_graph(False)
+        out["leaves_4096"] = leaves_workload(fxn, ctx)
+        out["flat_2m"] = flat2m_workload(fxn, ctx)
     if args.serve_requests > 0:
         out["serving"] = serving(fxn, ctx, args.serve_requests, args.serve_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(circ, x0, y0, G, args.density_weight, args.cpu_seconds)
         out["cpu_baseline"]["ratio_gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        if args.cpu_threads > 1:
+            mt = cpu_baseline(circ, x0, y0, G, args.density_weight, args.cpu_seconds,
+                              args.cpu_threads)
+            mt["ratio_gpu_over_cpu"] = value / mt["value"]
+            out["cpu_baseline_all_cores"] = mt
     job.close()
     ctx.close()
     print(json.dumps(out), flush=True)

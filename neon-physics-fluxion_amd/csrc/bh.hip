@@ -127,8 +127,7 @@ __global__ void bh_geometry(const float4* __restrict__ part, int nparts, int n,
   g->depth = d;
   g->scale_bits = sb;
   g->scale = ldexp(1.0, sb);
-  g->num_nodes = 0;
-  g->overflow = 0;
+  g->num_nodes = 0;  // overflow stays set for the rest of the run (cleared by the host)
 }
 
 // Quadrant path of each particle (insertIntoChild, BarnesHutTree.cpp:100-134).
@@ -503,7 +502,8 @@ int bh_alloc(Job& j) {
       (rc = b.owned.alloc(c, 4 * n)) || (rc = b.offset.alloc(c, 4 * (n + 1))) ||
       (rc = b.sxp.alloc(c, 16 * n)) || (rc = b.syp.alloc(c, 16 * (n + 1) + 16 * scan_elems)) ||
       (rc = b.fx.alloc(c, 4 * n)) || (rc = b.fy.alloc(c, 4 * n)) ||
-      (rc = b.stats.alloc(c, 64)) || (rc = b.sort.ensure(c, n)))
+      (rc = b.stats.alloc(c, 64)) || (rc = b.sort.ensure(c, j.stream, n)) ||
+      (rc = fill_i32(j.stream, b.geom.as<int32_t>(), sizeof(BhGeometry) / 4, 0)))
     return rc;
   // Lanes per particle in the walk. Measured on MI355X over the 500-iteration
   // 200k-gate run: W=1 0.185 ms, W=2 0.206 ms, W=4 0.356 ms per walk.
@@ -513,6 +513,12 @@ int bh_alloc(Job& j) {
     if (w == 1 || w == 2 || w == 4) b.walk_lanes = w;
   }
   return bh_grow(j, std::max<int64_t>(3 * n, n + 64 * 33) + 1024);
+}
+
+int bh_clear_overflow(Job& j) {
+  if (!j.bh.geom.p) return FX_OK;
+  int32_t* flag = reinterpret_cast<int32_t*>(j.bh.geom.as<char>() + offsetof(BhGeometry, overflow));
+  return fill_i32(j.stream, flag, 1, 0);
 }
 
 int bh_grow(Job& j, int64_t needed) {
@@ -587,7 +593,9 @@ extern "C" int fx_job_bh_stats(fx_job* job, int64_t* stats) {
   fx::BhState& b = j.bh;
   for (int attempt = 0; attempt < 3; ++attempt) {
     FX_HIP(hipMemsetAsync(b.stats.p, 0, 64, j.stream));
-    int rc = fx::bh_build(j);
+    int rc = fx::bh_clear_overflow(j);
+    if (rc) return rc;
+    rc = fx::bh_build(j);
     if (rc) return rc;
     if ((rc = fx::launch_walk<true>(j, b.stats.as<unsigned long long>()))) return rc;
     fx::BhGeometry g;

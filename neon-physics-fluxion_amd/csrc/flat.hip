@@ -542,11 +542,14 @@ static int snapshot(Job& j, bool restore) {
   return FX_OK;
 }
 
+int restore_snapshot(Job& j) { return snapshot(j, true); }
+
 int run_job(Job& j, int iters, bool wait) {
   if (iters <= 0) return FX_OK;
   for (int attempt = 0; attempt < 4; ++attempt) {
     int rc;
     if ((rc = snapshot(j, false))) return rc;
+    if ((job_components(j) & 2) && (rc = bh_clear_overflow(j))) return rc;
     if ((rc = enqueue_iterations(j, iters))) return rc;
     if (!wait) return FX_OK;
     bool rerun = false;
@@ -752,6 +755,13 @@ extern "C" int fx_job_sync(fx_job* job) {
   bool rerun = false;
   int rc = fx::check_regrow(job->j, &rerun);
   if (rc) return rc;
+  if (rerun) {
+    // The captured graph holds the old node buffer: capture again on the next
+    // run. Positions and velocities go back to the run's start.
+    fx::drop_graph(job->j);
+    if ((rc = fx::restore_snapshot(job->j))) return rc;
+    FX_HIP(hipStreamSynchronize(job->j.stream));
+  }
   if (rerun)
     return fx::fail(FX_E_STATE,
                     "Barnes-Hut node storage was grown during an asynchronous run; the state "
@@ -774,8 +784,9 @@ extern "C" int fx_job_forces(fx_job* job, int32_t components, float* fx_out, flo
   int comps = components & 7;
   if (j.p.density_grid <= 0) comps &= ~4;
   for (int attempt = 0; attempt < 3; ++attempt) {
-    int rc = fx::enqueue_phases(j, comps, false);
-    if (rc) return rc;
+    int rc;
+    if ((comps & 2) && (rc = fx::bh_clear_overflow(j))) return rc;
+    if ((rc = fx::enqueue_phases(j, comps, false))) return rc;
     bool rerun = false;
     if ((comps & 2) && (rc = fx::check_regrow(j, &rerun))) return rc;
     if (!rerun) break;

@@ -65,7 +65,9 @@ struct SortWorkspace {
   DevBuf keys_alt, vals_alt, hist;
   int64_t capacity = 0;
   int64_t blocks = 0;
-  int ensure(fx_ctx* ctx, int64_t n);
+  uint64_t spin_ticks = 0;  // look-back time limit in wall-clock ticks
+  // Allocates for n items and clears the look-back state on `st`.
+  int ensure(fx_ctx* ctx, hipStream_t st, int64_t n);
 };
 // Stable LSD radix sort over key bits [lo_bit, hi_bit). keys/vals are sorted
 // in place (ping-pong through the workspace). Enqueued on `stream`.

@@ -101,6 +101,9 @@ int combine(Job& j, int components, bool integrate);  // forces -> (fxo,fyo) or 
 int bh_alloc(Job& j);
 int dct_alloc(Job& j);
 int bh_grow(Job& j, int64_t needed);
+// Clears the overflow flag (set by any iteration of a run that exceeded the
+// node capacity; stays set until the next run starts).
+int bh_clear_overflow(Job& j);
 
 }  // namespace fx
 

@@ -13,8 +13,12 @@
 // 64-lane wave, eight ballots match lanes with equal digits — and scatters
 // them. Status words of the next pass are cleared by the current one
 // (double buffering), so nothing needs a memset inside a captured graph.
-// The look-back spin is bounded: on timeout a flag is raised and the result
-// is marked invalid instead of the GPU hanging.
+// The look-back wait is bounded in time (wall clock): on timeout a flag is
+// raised, no item of a timed-out tile is written, every scatter index is
+// checked against n, and a last kernel replaces the output by the identity
+// permutation, so a failed sort can neither hang nor write out of bounds and
+// its consumers still index in range; the host reads the flag and reports
+// the error.
 
 #include "internal.h"
 
@@ -24,7 +28,7 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kStAgg = 1u << 30, kStInc = 2u << 30, kStMask = (1u << 30) - 1;
-constexpr int kSpinLimit = 1 << 22;
+constexpr double kLookBackSeconds = 0.2;
 
 // Zeroes the 8x256 global digit counts and the per-pass tile counters.
 __global__ void radix_reset(uint32_t* __restrict__ ghist, uint32_t* __restrict__ counters) {
@@ -74,16 +78,25 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
     int ntiles, const uint32_t* __restrict__ gbase, uint32_t* __restrict__ status,
     uint32_t* __restrict__ status_next, uint32_t* __restrict__ counter,
-    uint32_t* __restrict__ fault, uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    uint32_t* __restrict__ fault, uint64_t spin_ticks, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout) {
   __shared__ uint32_t cnt[256];
   __shared__ uint32_t base[256];
   __shared__ uint32_t wcount[4][256];
   __shared__ int tile_s;
+  __shared__ int timed_out;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  if (t == 0) tile_s = (int)atomicAdd(counter, 1u);
+  if (t == 0) {
+    tile_s = (int)atomicAdd(counter, 1u);
+    timed_out = 0;
+  }
   cnt[t] = 0;
   __syncthreads();
   const int tile = tile_s;
+  if (tile >= ntiles) {  // more ids than workgroups: the counter was not reset
+    if (t == 0) atomicOr(fault, 2u);
+    return;
+  }
   status_next[(int64_t)tile * 256 + t] = 0;
   uint64_t k[kSortItems];
   uint32_t v[kSortItems];
@@ -109,15 +122,18 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
   } else {
     __hip_atomic_store(st, kStAgg | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t excl = 0;
-    int b = tile - 1, spins = 0;
+    int b = tile - 1;
+    const uint64_t start = wall_clock64();
     while (b >= 0) {
       const uint32_t w = __hip_atomic_load(status + (int64_t)b * 256 + t, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
       if ((w & ~kStMask) == 0) {
-        if (++spins > kSpinLimit) {
+        if (wall_clock64() - start > spin_ticks) {
           atomicOr(fault, 1u);
+          timed_out = 1;
           break;
         }
+        __builtin_amdgcn_s_sleep(1);
         continue;
       }
       excl += w & kStMask;
@@ -127,6 +143,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     __hip_atomic_store(st, kStInc | (excl + mine), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     base[t] = gbase[t] + excl;
   }
+  __syncthreads();
+  const bool write_ok = timed_out == 0;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
@@ -153,14 +171,26 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     const uint32_t pos = base[d] + off + rank;
     __syncthreads();
     base[t] += wcount[0][t] + wcount[1][t] + wcount[2][t] + wcount[3][t];
-    if (valid) {
+    if (valid && write_ok && (int64_t)pos < n) {
       kout[pos] = k[r];
       vout[pos] = v[r];
     }
   }
 }
 
-int SortWorkspace::ensure(fx_ctx* ctx, int64_t n) {
+// After a failed sort (fault flag set): identity permutation and zero keys,
+// so that consumers of the permutation index in range.
+__global__ void radix_repair(const uint32_t* __restrict__ fault, uint64_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals, int64_t n) {
+  if (*fault == 0) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    keys[i] = 0;
+    vals[i] = (uint32_t)i;
+  }
+}
+
+int SortWorkspace::ensure(fx_ctx* ctx, hipStream_t st, int64_t n) {
   if (n <= capacity) return FX_OK;
   const int64_t nt = (n + kSortTile - 1) / kSortTile;
   int rc;
@@ -169,7 +199,16 @@ int SortWorkspace::ensure(fx_ctx* ctx, int64_t n) {
   if ((rc = keys_alt.alloc(ctx, sizeof(uint64_t) * n)) || (rc = vals_alt.alloc(ctx, sizeof(uint32_t) * n)) ||
       (rc = hist.alloc(ctx, sizeof(uint32_t) * words)))
     return rc;
-  FX_HIP(hipMemset(hist.p, 0, sizeof(uint32_t) * words));
+  // Cleared on the stream that sorts: job streams are non-blocking, so work
+  // on the null stream would not be ordered before the first sort.
+  if ((rc = fill_i32(st, hist.as<int32_t>(), (int64_t)words, 0))) return rc;
+  if (!spin_ticks) {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess ||
+        khz <= 0)
+      khz = 100000;
+    spin_ticks = (uint64_t)(kLookBackSeconds * 1e3 * khz);
+  }
   capacity = n;
   blocks = nt;
   return FX_OK;
@@ -178,7 +217,7 @@ int SortWorkspace::ensure(fx_ctx* ctx, int64_t n) {
 int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* keys,
                      uint32_t* vals, int64_t n, int lo_bit, int hi_bit) {
   if (n <= 1) return FX_OK;
-  int rc = ws.ensure(ctx, n);
+  int rc = ws.ensure(ctx, st, n);
   if (rc) return rc;
   const int nt = (int)((n + kSortTile - 1) / kSortTile);
   const int passes = (hi_bit - lo_bit + 7) / 8;
@@ -201,7 +240,7 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
     uint32_t* nxt = (p & 1) ? status0 : status1;
     hipLaunchKernelGGL(radix_onesweep, dim3(nt), dim3(kSortThreads), 0, st, ka, va, n,
                        lo_bit + 8 * p, nt, (const uint32_t*)(gbase + p * 256), cur, nxt,
-                       counters + p, fault, kb, vb);
+                       counters + p, fault, ws.spin_ticks, kb, vb);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
@@ -214,6 +253,9 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
     FX_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
     FX_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
   }
+  hipLaunchKernelGGL(radix_repair, dim3((unsigned)std::min<int64_t>(div_up(n, 256), 1024)),
+                     dim3(256), 0, st, (const uint32_t*)fault, keys, vals, n);
+  FX_HIP(hipGetLastError());
   return FX_OK;
 }
 
