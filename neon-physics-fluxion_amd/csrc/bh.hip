@@ -165,13 +165,23 @@ __device__ __forceinline__ int lcp_levels(const uint64_t* __restrict__ keys, int
   return l < D ? l : D;
 }
 
+// Levels of minima of L used to find where a cell ends: level 0 is L itself,
+// level k holds the minimum over 16^k consecutive positions. Levels 1-2 are
+// int8, levels 3-7 int32 (written with atomicMin), all padded with 127.
+struct MinLevels {
+  const int8_t* l1;
+  const int8_t* l2;
+  const int* hi;
+  int hi_stride;
+};
+
 // L[i], per-position node counts, fixed-point positions in sorted order and
-// the block minima of L (blocks of 256, super-blocks of 256 blocks).
+// the minima of L over 16, 256, 16^3, ... positions.
 __global__ __launch_bounds__(kBlock) void bh_lcp_count(
     int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const float* __restrict__ x, const float* __restrict__ y, const BhGeometry* __restrict__ g,
-    int8_t* __restrict__ L, int8_t* __restrict__ bmin1, int* __restrict__ bmin2,
-    int* __restrict__ owned, I64x2* __restrict__ fix) {
+    int8_t* __restrict__ L, int8_t* __restrict__ m1, int8_t* __restrict__ m2,
+    int* __restrict__ mhi, int hi_stride, int* __restrict__ owned, I64x2* __restrict__ fix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int D = g->depth;
   int li = 127;
@@ -187,75 +197,96 @@ __global__ __launch_bounds__(kBlock) void bh_lcp_count(
     fix[i].b = llrint(((double)y[p] - g->root[1]) * sc);
   }
   int m = li;
-  for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_down(m, off, 64));
+  for (int off = 1; off < 16; off <<= 1) m = min(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 15) == 0 && i < n) m1[i >> 4] = (int8_t)m;
+  for (int off = 16; off < 64; off <<= 1) m = min(m, __shfl_xor(m, off, 64));
   __shared__ int sh[kBlock / 64];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kBlock / 64; ++w) m = min(m, sh[w]);
-    bmin1[blockIdx.x] = (int8_t)m;
-    atomicMin(&bmin2[blockIdx.x >> 8], m);
+    const int b = blockIdx.x;
+    m2[b] = (int8_t)m;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) atomicMin(&mhi[k * hi_stride + (b >> (4 * (k + 1)))], m);
   }
 }
 
-// First index q in [s, e) with a[q] < d, else e. Reads 16 bytes at a time
-// once aligned (arrays are padded by 16 bytes).
-__device__ __forceinline__ int first_lt(const int8_t* __restrict__ a, int s, int e, int d) {
-  int q = s;
-  for (; q < e && (q & 15); ++q)
-    if (a[q] < d) return q;
-  for (; q < e; q += 16) {
-    const uint4 v = *reinterpret_cast<const uint4*>(a + q);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+// Bit j set when entry j of the 16 entries at p (16-aligned) is below d.
+__device__ __forceinline__ unsigned lt_mask8(const int8_t* __restrict__ p, int d) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  unsigned m = 0;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int b = (int)(int8_t)((w[k >> 2] >> (8 * (k & 3))) & 0xFFu);
-      if (b < d && q + k < e) return q + k;
-    }
+  for (int k = 0; k < 16; ++k) {
+    const int b = (int)(int8_t)((w[k >> 2] >> (8 * (k & 3))) & 0xFFu);
+    m |= (unsigned)(b < d) << k;
   }
-  return e;
+  return m;
+}
+
+__device__ __forceinline__ unsigned lt_mask32(const int* __restrict__ p, int d) {
+  const int4* q = reinterpret_cast<const int4*>(p);
+  const int4 a = q[0], b = q[1], c = q[2], e = q[3];
+  const int v[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w,
+                     c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
+  unsigned m = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) m |= (unsigned)(v[k] < d) << k;
+  return m;
+}
+
+__device__ __forceinline__ unsigned level_mask(const int8_t* __restrict__ L, const MinLevels& t,
+                                               int lvl, int start, int d) {
+  if (lvl == 0) return lt_mask8(L + start, d);
+  if (lvl == 1) return lt_mask8(t.l1 + start, d);
+  if (lvl == 2) return lt_mask8(t.l2 + start, d);
+  return lt_mask32(t.hi + (lvl - 3) * t.hi_stride + start, d);
 }
 
 // First position q >= pos with L[q] < d (L[n-1] = -1 ends every search).
 // Inside the workgroup's own 256 positions (block blk) the answer comes from
-// per-depth ballot masks; beyond it, from the block minima (256 positions
-// each) and the super-block minima (256 blocks each), 16 bytes per load.
-__device__ int find_lt(const int8_t* __restrict__ L, const int8_t* __restrict__ b1,
-                       const int* __restrict__ b2, int n, int nb1, int nb2,
-                       const unsigned long long (*masks)[33], int blk, int pos, int d) {
-  int b;
+// per-depth ballot masks in LDS; beyond it, the minima levels are climbed
+// until a group of 16 holds an entry below d and then descended, one
+// 16-entry load per level: at most 2 x 7 dependent loads.
+__device__ __forceinline__ int next_lt(const int8_t* __restrict__ L, const MinLevels& t, int n,
+                                       const unsigned long long (*masks)[33], int blk, int pos,
+                                       int d) {
+  int lvl = 0, idx = pos;
   if ((pos >> 8) == blk) {
     const int wave = (pos & 255) >> 6, lane = pos & 63;
     const unsigned long long m = masks[wave][d] & (~0ull << lane);
     if (m) return (pos & ~63) + __ffsll((long long)m) - 1;
     for (int w = wave + 1; w < 4; ++w)
       if (masks[w][d]) return (pos & ~255) + w * 64 + __ffsll((long long)masks[w][d]) - 1;
-    b = blk + 1;
-  } else {
-    const int e = min(n, ((pos >> 8) + 1) << 8);
-    const int q = first_lt(L, pos, e, d);
-    if (q < e) return q;
-    b = (pos >> 8) + 1;
+    lvl = 2;
+    idx = blk + 1;
   }
-  if (b >= nb1) return n - 1;
-  const int sb_end = min(nb1, ((b >> 8) + 1) << 8);
-  b = first_lt(b1, b, sb_end, d);
-  if (b == sb_end) {
-    int s = b >> 8;
-    while (s < nb2 && b2[s] >= d) ++s;
-    if (s >= nb2) return n - 1;
-    b = first_lt(b1, s << 8, min(nb1, (s + 1) << 8), d);
+  for (;;) {
+    const int g = idx & ~15;
+    const unsigned m = level_mask(L, t, lvl, g, d) & (0xFFFFu << (idx & 15));
+    if (m) {
+      idx = g + __ffs(m) - 1;
+      break;
+    }
+    idx = (idx >> 4) + 1;
+    if (++lvl > 7) return n - 1;  // unreachable: the entry holding n-1 is -1
   }
-  if (b >= nb1) return n - 1;
-  const int q = first_lt(L, b << 8, min(n, (b + 1) << 8), d);
-  return q < n ? q : n - 1;
+  while (lvl > 0) {
+    --lvl;
+    const int g = idx << 4;
+    const unsigned m = level_mask(L, t, lvl, g, d);
+    if (!m) return n - 1;  // unreachable for consistent levels
+    idx = g + __ffs(m) - 1;
+  }
+  return idx;
 }
 
 __global__ __launch_bounds__(kBlock) void bh_emit(
     int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-    const int8_t* __restrict__ L, const int8_t* __restrict__ bmin1, const int* __restrict__ bmin2,
-    int nb1, int nb2, const int* __restrict__ offset, const I64x2* __restrict__ fixpre,
-    BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap) {
+    const int8_t* __restrict__ L, MinLevels levels, const int* __restrict__ offset,
+    const I64x2* __restrict__ fixpre, BhGeometry* __restrict__ g, BhNode* __restrict__ nodes,
+    int64_t cap) {
   __shared__ unsigned long long masks[kBlock / 64][33];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -278,35 +309,28 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
   const int id0 = offset[i];
   const uint64_t key = keys[i];
   const double r0 = g->root[0], r1 = g->root[1], scale = g->scale;
-  // The cells starting here are nested: ends are found deepest first, each
-  // search starting at the end of the cell below it.
   const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
   const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
-  int ends[33];
-  {
-    int e = find_lt(L, bmin1, bmin2, n, nb1, nb2, masks, blockIdx.x, i, leaf_depth);
-    ends[internal] = e;
-    for (int k = internal - 1; k >= 0; --k) {
-      e = find_lt(L, bmin1, bmin2, n, nb1, nb2, masks, blockIdx.x, e, first_depth + k);
-      ends[k] = e;
-    }
-  }
-  // Walk down the key once, emitting the cells that start here.
+  // Cells starting here, shallowest first: internal cells at depths
+  // first_depth.., then the leaf. The walk down the key gives each internal
+  // cell its bounds (a leaf stores its particle instead of its size).
   double xl = g->root[0], yl = g->root[1], xh = g->root[2], yh = g->root[3];
   int depth = 0;
   for (int k = 0; k <= internal; ++k) {
     const bool is_leaf = (k == internal);
     const int d = is_leaf ? leaf_depth : first_depth + k;
-    while (depth < d) {
-      const double mx = (xl + xh) / 2, my = (yl + yh) / 2;
-      const int quad = (int)((key >> (62 - 2 * depth)) & 3ull);
-      if (quad == 0 || quad == 2) xh = mx; else xl = mx;
-      if (quad >= 2) yh = my; else yl = my;
-      ++depth;
+    if (!is_leaf) {
+      while (depth < d) {
+        const double mx = (xl + xh) / 2, my = (yl + yh) / 2;
+        const int quad = (int)((key >> (62 - 2 * depth)) & 3ull);
+        if (quad == 0 || quad == 2) xh = mx; else xl = mx;
+        if (quad >= 2) yh = my; else yl = my;
+        ++depth;
+      }
     }
     const int id = id0 + k;
     if ((int64_t)id >= cap) continue;
-    const int end = ends[k];
+    const int end = next_lt(L, levels, n, masks, blockIdx.x, i, d);
     const int count = end - i + 1;
     const long long sx = fixpre[end + 1].a - fixpre[i].a;
     const long long sy = fixpre[end + 1].b - fixpre[i].b;
@@ -486,24 +510,32 @@ int launch_walk(Job& j, unsigned long long* stats) {
 
 }  // namespace
 
+// Bytes for an int8 level of `c` entries: whole 16-entry groups plus one spare group.
+static int64_t pad16(int64_t c) { return (c + 15) / 16 * 16 + 16; }
+
 int bh_alloc(Job& j) {
   if ((j.p.flags & FX_FLAG_NO_REPULSION)) return FX_OK;
   BhState& b = j.bh;
   const int64_t n = j.N;
   b.nb1 = (n + 255) / 256;
-  b.nb2 = (b.nb1 + 255) / 256;
+  const int64_t c1 = (n + 15) / 16;
+  b.hi_stride = ((n + 4095) / 4096 + 15) / 16 * 16 + 16;
   const int64_t scan_elems = scan_tmp_elems(n + 1);
   int rc;
   fx_ctx* c = j.ctx;
   if ((rc = b.part.alloc(c, sizeof(float4) * kReduceBlocks)) ||
       (rc = b.geom.alloc(c, sizeof(BhGeometry))) || (rc = b.keys.alloc(c, 8 * n)) ||
-      (rc = b.vals.alloc(c, 4 * n)) || (rc = b.lcp.alloc(c, n + 16)) ||
-      (rc = b.bmin1.alloc(c, b.nb1 + 32)) || (rc = b.bmin2.alloc(c, 4 * b.nb2)) ||
+      (rc = b.vals.alloc(c, 4 * n)) || (rc = b.lcp.alloc(c, pad16(n))) ||
+      (rc = b.m1.alloc(c, pad16(c1))) || (rc = b.m2.alloc(c, pad16(b.nb1))) ||
+      (rc = b.mhi.alloc(c, 4 * 5 * b.hi_stride)) ||
       (rc = b.owned.alloc(c, 4 * n)) || (rc = b.offset.alloc(c, 4 * (n + 1))) ||
       (rc = b.sxp.alloc(c, 16 * n)) || (rc = b.syp.alloc(c, 16 * (n + 1) + 16 * scan_elems)) ||
       (rc = b.fx.alloc(c, 4 * n)) || (rc = b.fy.alloc(c, 4 * n)) ||
       (rc = b.stats.alloc(c, 64)) || (rc = b.sort.ensure(c, j.stream, n)) ||
-      (rc = fill_i32(j.stream, b.geom.as<int32_t>(), sizeof(BhGeometry) / 4, 0)))
+      (rc = fill_i32(j.stream, b.geom.as<int32_t>(), sizeof(BhGeometry) / 4, 0)) ||
+      (rc = fill_i32(j.stream, b.lcp.as<int32_t>(), pad16(n) / 4, 0x7F7F7F7F)) ||
+      (rc = fill_i32(j.stream, b.m1.as<int32_t>(), pad16(c1) / 4, 0x7F7F7F7F)) ||
+      (rc = fill_i32(j.stream, b.m2.as<int32_t>(), pad16(b.nb1) / 4, 0x7F7F7F7F)))
     return rc;
   // Lanes per particle in the walk. Measured on MI355X over the 500-iteration
   // 200k-gate run: W=1 0.185 ms, W=2 0.206 ms, W=4 0.356 ms per walk.
@@ -544,7 +576,7 @@ static int bh_build(Job& j) {
                      j.x.as<float>(), j.y.as<float>(), b.part.as<float4>());
   hipLaunchKernelGGL(bh_geometry, dim3(1), dim3(64), 0, st, b.part.as<float4>(), kReduceBlocks, n, g);
   FX_HIP(hipGetLastError());
-  if ((rc = fill_i32(st, b.bmin2.as<int32_t>(), b.nb2, 127))) return rc;
+  if ((rc = fill_i32(st, b.mhi.as<int32_t>(), 5 * b.hi_stride, 127))) return rc;
   hipLaunchKernelGGL(bh_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n, j.x.as<float>(),
                      j.y.as<float>(), (const BhGeometry*)g, b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
   FX_HIP(hipGetLastError());
@@ -554,8 +586,8 @@ static int bh_build(Job& j) {
   if ((rc = phase_end(j, kPhaseBhSort)) || (rc = phase_begin(j, kPhaseBhTree))) return rc;
   hipLaunchKernelGGL(bh_lcp_count, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
-                     (const BhGeometry*)g, b.lcp.as<int8_t>(), b.bmin1.as<int8_t>(), b.bmin2.as<int>(),
-                     b.owned.as<int>(), b.sxp.as<I64x2>());
+                     (const BhGeometry*)g, b.lcp.as<int8_t>(), b.m1.as<int8_t>(), b.m2.as<int8_t>(),
+                     b.mhi.as<int>(), (int)b.hi_stride, b.owned.as<int>(), b.sxp.as<I64x2>());
   FX_HIP(hipGetLastError());
   const int64_t scan_elems = scan_tmp_elems(n + 1);
   I64x2* fixpre = b.syp.as<I64x2>();
@@ -565,9 +597,13 @@ static int bh_build(Job& j) {
                                reinterpret_cast<int32_t*>(tmp), scan_elems)))
     return rc;
   if ((rc = scan_exclusive_i64x2(st, b.sxp.p, fixpre, n, tmp, scan_elems))) return rc;
+  MinLevels lv;
+  lv.l1 = b.m1.as<int8_t>();
+  lv.l2 = b.m2.as<int8_t>();
+  lv.hi = b.mhi.as<int>();
+  lv.hi_stride = (int)b.hi_stride;
   hipLaunchKernelGGL(bh_emit, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
-                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), b.lcp.as<int8_t>(),
-                     b.bmin1.as<int8_t>(), b.bmin2.as<int>(), (int)b.nb1, (int)b.nb2,
+                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), b.lcp.as<int8_t>(), lv,
                      b.offset.as<int>(), (const I64x2*)fixpre, g, b.nodes.as<BhNode>(), b.node_cap);
   FX_HIP(hipGetLastError());
   return phase_end(j, kPhaseBhTree);

@@ -27,13 +27,17 @@ struct __align__(32) BhNode {
 };
 
 struct BhState {
-  DevBuf part, geom, keys, vals, lcp, bmin1, bmin2, owned, offset, sxp, syp, nodes, fx, fy,
+  // lcp: L[i] = common levels of keys i and i+1 (int8, -1 at n-1); m1, m2: minima of L over
+  // 16 and 256 positions (int8); mhi: minima over 16^3 .. 16^7 positions (int32, one level
+  // every hi_stride entries). Padding entries read 127 so searches need no bounds.
+  DevBuf part, geom, keys, vals, lcp, m1, m2, mhi, owned, offset, sxp, syp, nodes, fx, fy,
       stats;
   SortWorkspace sort;
   bool sorted_valid = false;  // vals holds a permutation from a completed build
   int walk_lanes = 1;         // lanes per particle in the tree walk (1, 2, 4)
   int64_t node_cap = 0;
-  int64_t nb1 = 0, nb2 = 0;
+  int64_t nb1 = 0;       // 256-position blocks (workgroups of the tree kernels)
+  int64_t hi_stride = 0;  // entries per level in mhi
 };
 
 struct DctState {
