@@ -32,6 +32,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include "job.h"
 
@@ -127,6 +128,7 @@ __global__ void bh_geometry(const float4* __restrict__ part, int nparts, int n,
   g->depth = d;
   g->scale_bits = sb;
   g->scale = ldexp(1.0, sb);
+  g->size_f = (float)s_root;
   g->num_nodes = 0;  // overflow stays set for the rest of the run (cleared by the host)
 }
 
@@ -283,10 +285,9 @@ __device__ __forceinline__ int next_lt(const int8_t* __restrict__ L, const MinLe
 }
 
 __global__ __launch_bounds__(kBlock) void bh_emit(
-    int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-    const int8_t* __restrict__ L, MinLevels levels, const int* __restrict__ offset,
-    const I64x2* __restrict__ fixpre, BhGeometry* __restrict__ g, BhNode* __restrict__ nodes,
-    int64_t cap) {
+    int n, const uint32_t* __restrict__ vals, const int8_t* __restrict__ L, MinLevels levels,
+    const int* __restrict__ offset, const I64x2* __restrict__ fixpre,
+    BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap) {
   __shared__ unsigned long long masks[kBlock / 64][33];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -307,43 +308,31 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
   if (lp >= D) return;  // continuation of a merged leaf
   const int internal = max(0, min(li, D - 1) - lp);
   const int id0 = offset[i];
-  const uint64_t key = keys[i];
   const double r0 = g->root[0], r1 = g->root[1], scale = g->scale;
   const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
   const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
   // Cells starting here, shallowest first: internal cells at depths
-  // first_depth.., then the leaf. The walk down the key gives each internal
-  // cell its bounds (a leaf stores its particle instead of its size).
-  double xl = g->root[0], yl = g->root[1], xh = g->root[2], yh = g->root[3];
-  int depth = 0;
+  // first_depth.., then the leaf.
   for (int k = 0; k <= internal; ++k) {
     const bool is_leaf = (k == internal);
     const int d = is_leaf ? leaf_depth : first_depth + k;
-    if (!is_leaf) {
-      while (depth < d) {
-        const double mx = (xl + xh) / 2, my = (yl + yh) / 2;
-        const int quad = (int)((key >> (62 - 2 * depth)) & 3ull);
-        if (quad == 0 || quad == 2) xh = mx; else xl = mx;
-        if (quad >= 2) yh = my; else yl = my;
-        ++depth;
-      }
-    }
     const int id = id0 + k;
     if ((int64_t)id >= cap) continue;
     const int end = next_lt(L, levels, n, masks, blockIdx.x, i, d);
     const int count = end - i + 1;
     const long long sx = fixpre[end + 1].a - fixpre[i].a;
     const long long sy = fixpre[end + 1].b - fixpre[i].b;
-    float4 nd;
+    BhNode nd;
     nd.x = (float)(r0 + ((double)sx / count) / scale);
     nd.y = (float)(r1 + ((double)sy / count) / scale);
-    nd.z = is_leaf ? __int_as_float((int)vals[i]) : (float)fmax(xh - xl, yh - yl);
-    nd.w = is_leaf ? -(float)count : (float)count;
-    BhNode rec;
-    rec.a = nd;
-    rec.skip = (end + 1 < n) ? offset[end + 1] : M;
-    rec.pad0 = rec.pad1 = rec.pad2 = 0;
-    nodes[id] = rec;
+    if (is_leaf) {
+      nd.z = (int)vals[i];
+      nd.w = (int)(0x80000000u | (uint32_t)count);
+    } else {
+      nd.z = (end + 1 < n) ? offset[end + 1] : M;
+      nd.w = (d << 26) | count;
+    }
+    nodes[id] = nd;
   }
 }
 
@@ -357,29 +346,50 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
 // quadrant; the four are added in quadrant order, as the path of record
 // does, so every W gives the same bits. More lanes per particle put more
 // waves in flight to hide the latency of the dependent node loads.
+// Parameters of the force law and of the opening test.
+struct WalkParams {
+  float k, theta, min_dist;
+  // Exact opening test without a division: for a, b > 0, fl(a/b) < theta iff
+  // a < theta_m * b with theta_m the midpoint between theta and its
+  // predecessor (the rounding boundary; the product is exact in FP64), or
+  // a == theta_m * b when that tie rounds down (theta_tie_le; a/b can equal
+  // theta_m only for subnormal theta).
+  double theta_m;
+  int theta_tie_le;
+};
+
 // Per-node interaction of the walk. Exact: the reference's operations with
-// IEEE division and square root (bit-identical to the FP32 path of record).
+// IEEE square root and division (bit-identical to the FP32 path of record;
+// the opening test is the same predicate evaluated exactly in FP64).
 // Fast: the same law with v_rsq_f32 and FMA and the opening test squared
 // (size^2 < theta^2 max(d^2, min_dist^2)); forces within 1e-6 relative.
 template <bool kFast>
-__device__ __forceinline__ bool bh_node(const float4& nd, bool leaf, float px, float py, float k,
-                                        float theta, float min_dist, float& ax, float& ay) {
+__device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, float py,
+                                        const WalkParams& wp, float size_root, float& ax,
+                                        float& ay) {
   const float dx = px - nd.x, dy = py - nd.y;
-  const float q = leaf ? -nd.w : nd.w;
+  const float q = (float)(nd.w & (leaf ? 0x7FFFFFFF : kBhMaxCount));
+  const float size = ldexpf(size_root, -((nd.w >> 26) & 31));
   if (kFast) {
-    const float d2 = fmaxf(fmaf(dx, dx, dy * dy), min_dist * min_dist);
-    if (!leaf && !(nd.z * nd.z < theta * theta * d2)) return false;
+    const float d2 = fmaxf(fmaf(dx, dx, dy * dy), wp.min_dist * wp.min_dist);
+    if (!leaf && !(size * size < wp.theta * wp.theta * d2)) return false;
     const float r = __builtin_amdgcn_rsqf(d2);
-    const float f = k * q * (r * r * r);
+    const float f = wp.k * q * (r * r * r);
     ax = fmaf(f, dx, ax);
     ay = fmaf(f, dy, ay);
     return true;
   }
   const float dist_sq = dx * dx + dy * dy;
   float dist = sqrtf(dist_sq);
-  if (dist < min_dist) dist = min_dist;
-  if (leaf || (nd.z / dist < theta)) {
-    const float f = (k * 1.0f * q) / (dist * dist);
+  if (dist < wp.min_dist) dist = wp.min_dist;
+  bool accept = leaf;
+  if (!leaf) {
+    const double t = wp.theta_m * (double)dist;
+    const double a = (double)size;
+    accept = a < t || (wp.theta_tie_le && a == t);  // == (size / dist < theta)
+  }
+  if (accept) {
+    const float f = (wp.k * 1.0f * q) / (dist * dist);
     ax += f * (dx / dist);
     ay += f * (dy / dist);
     return true;
@@ -387,13 +397,30 @@ __device__ __forceinline__ bool bh_node(const float4& nd, bool leaf, float px, f
   return false;
 }
 
+// One 16-byte load of a node (a single vector load: the compiler must not
+// split it into a load of (z, w) and a dependent load of (x, y)).
+__device__ __forceinline__ BhNode load_node(const BhNode* __restrict__ nodes, int c) {
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  const v4i v = *(reinterpret_cast<const v4i*>(nodes) + c);
+  BhNode nd;
+  nd.x = __int_as_float(v.x);
+  nd.y = __int_as_float(v.y);
+  nd.z = v.z;
+  nd.w = v.w;
+  return nd;
+}
+
+// Index of the first node after the subtree of node c.
+__device__ __forceinline__ int node_next(const BhNode& nd, int c) {
+  return nd.w < 0 ? c + 1 : nd.z;
+}
+
 template <bool kStats, int W, bool kFast>
 __global__ __launch_bounds__(kBlock) void bh_traverse(
     int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
     const float* __restrict__ y, const BhGeometry* __restrict__ g,
-    const BhNode* __restrict__ nodes, float k, float theta,
-    float min_dist, float* __restrict__ fx, float* __restrict__ fy,
-    unsigned long long* __restrict__ stats) {
+    const BhNode* __restrict__ nodes, WalkParams wp, float* __restrict__ fx,
+    float* __restrict__ fy, unsigned long long* __restrict__ stats) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = gid / W, r = gid % W;
   const bool valid = t < n;
@@ -409,14 +436,14 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
   unsigned long long inter = 0, opened = 0;
   if (valid) {
     const float px = x[p], py = y[p];
-    const BhNode rn = nodes[0];
-    const float4 root = rn.a;
-    const bool root_leaf = root.w < 0.0f;
-    const int root_skip = rn.skip;
+    const float size_root = g->size_f;
+    const BhNode root = load_node(nodes, 0);
+    const bool root_leaf = root.w < 0;
+    const int root_skip = root_leaf ? 1 : root.z;
     bool split = false;
-    if (!(root_leaf && __float_as_int(root.z) == p)) {
+    if (!(root_leaf && root.z == p)) {
       float rx = 0.0f, ry = 0.0f;
-      if (bh_node<kFast>(root, root_leaf, px, py, k, theta, min_dist, rx, ry)) {
+      if (bh_node<kFast>(root, root_leaf, px, py, wp, size_root, rx, ry)) {
         if (r == 0) {
           qx[0] = rx;
           qy[0] = ry;
@@ -429,28 +456,27 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
     }
     if (split) {
       int c = 1;
-      for (int j = 0; j < r && c < root_skip; ++j) c = nodes[c].skip;
+      for (int j = 0; j < r && c < root_skip; ++j) c = node_next(nodes[c], c);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (q % W != r) continue;
         if (c >= root_skip) break;
         int cur = c;
-        const int end = nodes[c].skip;
+        const int end = node_next(nodes[c], c);
         float ax = 0.0f, ay = 0.0f;
         while (cur < end) {
-          const BhNode rec = nodes[cur];
-          const float4 nd = rec.a;
-          const bool leaf = nd.w < 0.0f;
-          bool move_on = leaf && __float_as_int(nd.z) == p;
+          const BhNode nd = load_node(nodes, cur);
+          const bool leaf = nd.w < 0;
+          bool move_on = leaf && nd.z == p;
           if (!move_on) {
-            move_on = bh_node<kFast>(nd, leaf, px, py, k, theta, min_dist, ax, ay);
+            move_on = bh_node<kFast>(nd, leaf, px, py, wp, size_root, ax, ay);
             if (kStats) {
               if (move_on) ++inter;
               else ++opened;
             }
           }
           if (move_on) {
-            const int nxt = rec.skip;
+            const int nxt = leaf ? cur + 1 : nd.z;
             if (nxt <= cur) break;  // malformed tree: never loop
             cur = nxt;
           } else {
@@ -460,7 +486,7 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
         qx[q] = ax;
         qy[q] = ay;
         // advance to this lane's next quadrant (W siblings further)
-        for (int j = 0; j < W && c < root_skip; ++j) c = nodes[c].skip;
+        for (int j = 0; j < W && c < root_skip; ++j) c = node_next(nodes[c], c);
       }
     }
   }
@@ -488,10 +514,26 @@ int launch_walk(Job& j, unsigned long long* stats) {
   const int W = b.walk_lanes;
   const dim3 grid(div_up((int64_t)W * j.N, kBlock)), block(kBlock);
   const bool fast = (j.p.flags & FX_FLAG_FAST_MATH) != 0;
+  WalkParams wp;
+  wp.k = j.p.repel_weight;
+  wp.theta = j.p.theta;
+  wp.min_dist = j.p.min_dist;
+  {
+    // Rounding boundary below theta: the midpoint of theta and its
+    // predecessor (for +inf, the FP32 overflow threshold); exact in FP64.
+    const float th = j.p.theta;
+    const float lo = std::nextafterf(th, -INFINITY);
+    const double hi = (std::isinf(th) && th > 0) ? std::ldexp(1.0, 128) : (double)th;
+    wp.theta_m = ((double)lo + hi) / 2;
+    // A tie rounds to the even neighbour: down when theta's significand is odd.
+    uint32_t bits;
+    std::memcpy(&bits, &th, 4);
+    wp.theta_tie_le = (std::isfinite(th) && (bits & 1u)) ? 1 : 0;
+  }
 #define FX_WALK_ARGS                                                                      \
   grid, block, 0, j.stream, j.N, b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(), \
       (const BhGeometry*)b.geom.p, b.nodes.as<BhNode>(),                                  \
-      j.p.repel_weight, j.p.theta, j.p.min_dist, b.fx.as<float>(), b.fy.as<float>(), stats
+      wp, b.fx.as<float>(), b.fy.as<float>(), stats
 #define FX_WALK(WW)                                                                      \
   if (fast) hipLaunchKernelGGL((bh_traverse<kStats, WW, true>), FX_WALK_ARGS);           \
   else hipLaunchKernelGGL((bh_traverse<kStats, WW, false>), FX_WALK_ARGS)
@@ -517,6 +559,8 @@ int bh_alloc(Job& j) {
   if ((j.p.flags & FX_FLAG_NO_REPULSION)) return FX_OK;
   BhState& b = j.bh;
   const int64_t n = j.N;
+  FX_CHECK(n <= kBhMaxCount, FX_E_INVALID,
+           "Barnes-Hut repulsion supports at most %d gates per job", kBhMaxCount);
   b.nb1 = (n + 255) / 256;
   const int64_t c1 = (n + 15) / 16;
   b.hi_stride = ((n + 4095) / 4096 + 15) / 16 * 16 + 16;
@@ -603,7 +647,7 @@ static int bh_build(Job& j) {
   lv.hi = b.mhi.as<int>();
   lv.hi_stride = (int)b.hi_stride;
   hipLaunchKernelGGL(bh_emit, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
-                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), b.lcp.as<int8_t>(), lv,
+                     b.vals.as<uint32_t>(), b.lcp.as<int8_t>(), lv,
                      b.offset.as<int>(), (const I64x2*)fixpre, g, b.nodes.as<BhNode>(), b.node_cap);
   FX_HIP(hipGetLastError());
   return phase_end(j, kPhaseBhTree);

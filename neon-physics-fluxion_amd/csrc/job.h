@@ -15,16 +15,19 @@ struct BhGeometry {
   int32_t scale_bits;
   int32_t num_nodes;  // M of the last build
   int32_t overflow;   // set when M exceeded the node capacity
+  float size_f;       // root cell size in FP32; a cell at depth d has size size_f * 2^-d
 };
 
-// Quadtree node in DFS pre-order, one 32-byte record (a single round trip
-// per visit of the walk): (cx, cy, s, q) — internal cell: s = cell size,
-// q = +count; leaf: s = the bits of its first particle's index, q = -count —
-// and the skip pointer (first node after the subtree).
-struct __align__(32) BhNode {
-  float4 a;
-  int32_t skip, pad0, pad1, pad2;
+// Quadtree node in DFS pre-order, one 16-byte record (a single gather per
+// visit of the walk): x, y = charge-weighted centre; z = internal cell: the
+// index of the first node after its subtree, leaf: its first particle;
+// w = leaf: 0x80000000 | count, internal cell: depth << 26 | count. A leaf's
+// successor is the next index; a cell's size is the root size * 2^-depth.
+struct __align__(16) BhNode {
+  float x, y;
+  int32_t z, w;
 };
+constexpr int32_t kBhMaxCount = (1 << 26) - 1;  // particles per job with repulsion
 
 struct BhState {
   // lcp: L[i] = common levels of keys i and i+1 (int8, -1 at n-1); m1, m2: minima of L over

@@ -600,6 +600,7 @@ ORC_API void orc_bh_record_f32(int32_t n, const float* x, const float* y, float 
   int32_t D, sbits;
   orc_bh_geometry(n, x, y, root, &D, &sbits);
   const double scale = std::ldexp(1.0, sbits);
+  const float size_root_f = (float)std::max(root[2] - root[0], root[3] - root[1]);
   std::vector<RecNode> nodes;
   nodes.reserve(2 * (size_t)n + 16);
   RecNode r;
@@ -665,7 +666,10 @@ ORC_API void orc_bh_record_f32(int32_t n, const float* x, const float* y, float 
     nd.cx = (float)(root[0] + ((double)nd.sx / nd.count) / scale);
     nd.cy = (float)(root[1] + ((double)nd.sy / nd.count) / scale);
     nd.q = (float)nd.count;
-    nd.size = (float)std::max(nd.x_max - nd.x_min, nd.y_max - nd.y_min);
+    // Cell size max(w, h) (BarnesHutTree.cpp, BHNode::size) halves at every
+    // level in real arithmetic; taken as the FP32 root size times 2^-depth
+    // (exact), which the GPU tree derives from the depth it stores.
+    nd.size = std::ldexp(size_root_f, -nd.depth);
   }
   int64_t inter = 0, opened = 0;
 #ifdef _OPENMP

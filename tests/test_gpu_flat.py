@@ -76,24 +76,28 @@ def test_hpwl_kats_and_parity(ctx):
     assert h == pytest.approx(r, rel=1e-12)
 
 
-@pytest.mark.parametrize("N", [1, 2, 1000, 20000, 200000])
-def test_barnes_hut_vs_record(N):
-    print(f"case: BH N={N}", flush=True)
-    circ, x, y, job = make(max(N, 2), max(N, 2), 41 + N, density_grid=0)
+@pytest.mark.parametrize("N,theta", [(1, 0.5), (2, 0.5), (1000, 0.5), (20000, 0.5), (200000, 0.5),
+                                     (20000, 0.3), (20000, 0.7), (20000, 1.0), (20000, 0.0)])
+def test_barnes_hut_vs_record(N, theta):
+    # Exact walk == the FP32 path of record bit for bit, at theta values whose
+    # rounding boundary ties go up (0.5, 0.3, 1.0) and down (0.7), and at
+    # theta = 0 (every cell opened: the brute-force sum in tree order).
+    print(f"case: BH N={N} theta={theta}", flush=True)
+    circ, x, y, job = make(max(N, 2), max(N, 2), 41 + N, density_grid=0, theta=theta)
     if N == 1:
         circ = fxn.Circuit.from_nets(1, [[0]])
         job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig())
         x, y = np.array([3.0], np.float32), np.array([4.0], np.float32)
         job.set_positions(x, y)
     gx, gy = job.forces(2)
-    rx, ry, st = orc.bh_record(x, y, nthreads=8)
+    rx, ry, st = orc.bh_record(x, y, theta=theta, nthreads=8)
     if N == 1:
         assert gx[0] == 0 and gy[0] == 0
         return
-    err = max(nrm_err(gx, rx), nrm_err(gy, ry))
     same = np.mean((gx == rx) & (gy == ry))
-    print(f"BH N={N}: err {err:.2e}, bit-identical fraction {same:.4f}", flush=True)
-    assert err < 1e-4
+    print(f"BH N={N}: err {max(nrm_err(gx, rx), nrm_err(gy, ry)):.2e}, "
+          f"bit-identical fraction {same:.6f}", flush=True)
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
     s = job.bh_stats()
     assert s["nodes"] == st["nodes"] and s["merge_depth"] == st["merge_depth"]
     assert s["interactions"] == st["interactions"] and s["opened"] == st["opened"]
