@@ -38,19 +38,52 @@
 
 namespace fx {
 
-int64_t scan_tmp_elems(int64_t n);
-
 namespace {
 
 constexpr int kBlock = 256;
 
-struct I64x2 {
+// Per-position tree data summed by prefix scans: a, b = fixed-point x, y;
+// c = nodes owned by the position (internal cells starting there + its leaf).
+struct TreeSum {
   long long a, b;
+  int c, pad;
 };
 
+__device__ __forceinline__ TreeSum ts_add(const TreeSum& x, const TreeSum& y) {
+  TreeSum r;
+  r.a = x.a + y.a;
+  r.b = x.b + y.b;
+  r.c = x.c + y.c;
+  r.pad = 0;
+  return r;
+}
+
+// Inclusive scan over the 64 lanes of a wave.
+__device__ __forceinline__ TreeSum ts_wave_scan(TreeSum v, int lane) {
+  for (int off = 1; off < 64; off <<= 1) {
+    TreeSum u;
+    u.a = __shfl_up(v.a, off, 64);
+    u.b = __shfl_up(v.b, off, 64);
+    u.c = __shfl_up(v.c, off, 64);
+    u.pad = 0;
+    if (lane >= off) v = ts_add(v, u);
+  }
+  return v;
+}
+
+// Per-block bounding boxes of the positions. The same launch fills two
+// arrays for later kernels of the iteration (a, then b; b may be null).
 __global__ __launch_bounds__(kBlock) void bh_bbox_partial(int n, const float* __restrict__ x,
                                                           const float* __restrict__ y,
-                                                          float4* __restrict__ part) {
+                                                          float4* __restrict__ part,
+                                                          int32_t* __restrict__ fa, int64_t na,
+                                                          int32_t va, int32_t* __restrict__ fb,
+                                                          int64_t nb, int32_t vb) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < na + nb;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < na) fa[i] = va;
+    else fb[i - na] = vb;
+  }
   float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     mnx = fminf(mnx, x[i]);
@@ -83,11 +116,11 @@ __device__ __forceinline__ int bitlen_u64(unsigned long long v) {
   return v ? 64 - __clzll((long long)v) : 0;
 }
 
-// Root cell (BarnesHutTree.cpp:24-46), merge depth and fixed-point scale.
-__global__ void bh_geometry(const float4* __restrict__ part, int nparts, int n,
-                            BhGeometry* __restrict__ g) {
-  // One wave: lanes reduce strided partials, then a shuffle reduction.
-  const int lane = threadIdx.x;
+// Root cell (BarnesHutTree.cpp:24-46), merge depth and fixed-point scale,
+// from the per-block boxes; run by one whole wave, the result is in lane 0.
+__device__ void geometry_from_boxes(const float4* __restrict__ part, int nparts, int n,
+                                    BhGeometry& g) {
+  const int lane = threadIdx.x & 63;
   float4 r = part[lane < nparts ? lane : 0];
   for (int k = lane + 64; k < nparts; k += 64) {
     r.x = fminf(r.x, part[k].x);
@@ -121,21 +154,46 @@ __global__ void bh_geometry(const float4* __restrict__ part, int nparts, int n,
   const int s_bits = bitlen_u64(s_int > 0 ? s_int : 1ull);
   int sb = 62 - n_bits - s_bits;
   sb = sb < 0 ? 0 : (sb > 40 ? 40 : sb);
-  g->root[0] = root0;
-  g->root[1] = root1;
-  g->root[2] = root2;
-  g->root[3] = root3;
-  g->depth = d;
-  g->scale_bits = sb;
-  g->scale = ldexp(1.0, sb);
-  g->size_f = (float)s_root;
-  g->num_nodes = 0;  // overflow stays set for the rest of the run (cleared by the host)
+  g.root[0] = root0;
+  g.root[1] = root1;
+  g.root[2] = root2;
+  g.root[3] = root3;
+  g.depth = d;
+  g.scale_bits = sb;
+  g.scale = ldexp(1.0, sb);
+  g.size_f = (float)s_root;
 }
 
 // Quadrant path of each particle (insertIntoChild, BarnesHutTree.cpp:100-134).
-__global__ void bh_keys(int n, const float* __restrict__ x, const float* __restrict__ y,
-                        const BhGeometry* __restrict__ g, uint64_t* __restrict__ keys,
-                        uint32_t* __restrict__ vals) {
+// Every workgroup derives the root cell from the block boxes itself (the
+// same bits in each); workgroup 0 publishes it for the later kernels.
+__global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict__ x,
+                                                  const float* __restrict__ y,
+                                                  const float4* __restrict__ part, int nparts,
+                                                  BhGeometry* __restrict__ gout,
+                                                  uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals) {
+  __shared__ BhGeometry sg;
+  if (threadIdx.x < 64) {
+    BhGeometry gl;
+    geometry_from_boxes(part, nparts, n, gl);
+    if (threadIdx.x == 0) {
+      sg = gl;
+      if (blockIdx.x == 0) {
+        gout->root[0] = gl.root[0];
+        gout->root[1] = gl.root[1];
+        gout->root[2] = gl.root[2];
+        gout->root[3] = gl.root[3];
+        gout->depth = gl.depth;
+        gout->scale_bits = gl.scale_bits;
+        gout->scale = gl.scale;
+        gout->size_f = gl.size_f;
+        gout->num_nodes = 0;  // overflow stays set for the rest of the run
+      }
+    }
+  }
+  __syncthreads();
+  const BhGeometry* g = &sg;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double px = (double)x[i], py = (double)y[i];
@@ -183,21 +241,29 @@ __global__ __launch_bounds__(kBlock) void bh_lcp_count(
     int n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const float* __restrict__ x, const float* __restrict__ y, const BhGeometry* __restrict__ g,
     int8_t* __restrict__ L, int8_t* __restrict__ m1, int8_t* __restrict__ m2,
-    int* __restrict__ mhi, int hi_stride, int* __restrict__ owned, I64x2* __restrict__ fix) {
+    int* __restrict__ mhi, int hi_stride, TreeSum* __restrict__ tri,
+    TreeSum* __restrict__ tsum) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int D = g->depth;
   int li = 127;
+  TreeSum v = {0, 0, 0, 0};
   if (i < n) {
     li = lcp_levels(keys, n, i, D);
     const int lp = lcp_levels(keys, n, i - 1, D);
     L[i] = (int8_t)li;
     const int internal = max(0, min(li, D - 1) - lp);
-    owned[i] = (lp >= D) ? 0 : internal + 1;
+    v.c = (lp >= D) ? 0 : internal + 1;
     const uint32_t p = vals[i];
     const double sc = g->scale;
-    fix[i].a = llrint(((double)x[p] - g->root[0]) * sc);
-    fix[i].b = llrint(((double)y[p] - g->root[1]) * sc);
+    v.a = llrint(((double)x[p] - g->root[0]) * sc);
+    v.b = llrint(((double)y[p] - g->root[1]) * sc);
+    tri[i] = v;
   }
+  // Block total of the tree data (for the prefix scan of the block totals).
+  const TreeSum vin = ts_wave_scan(v, lane);
+  __shared__ TreeSum wtot[kBlock / 64];
+  if (lane == 63) wtot[threadIdx.x >> 6] = vin;
   int m = li;
   for (int off = 1; off < 16; off <<= 1) m = min(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 15) == 0 && i < n) m1[i >> 4] = (int8_t)m;
@@ -211,7 +277,55 @@ __global__ __launch_bounds__(kBlock) void bh_lcp_count(
     m2[b] = (int8_t)m;
 #pragma unroll
     for (int k = 0; k < 5; ++k) atomicMin(&mhi[k * hi_stride + (b >> (4 * (k + 1)))], m);
+    TreeSum tot = wtot[0];
+    for (int w = 1; w < kBlock / 64; ++w) tot = ts_add(tot, wtot[w]);
+    tsum[b] = tot;
   }
+}
+
+// Exclusive prefix of the per-block totals, in place (one workgroup).
+__global__ __launch_bounds__(1024) void tree_scan_sums(TreeSum* __restrict__ tsum, int nb) {
+  __shared__ TreeSum wtot[16];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int per = (nb + 1023) / 1024;
+  const int s0 = t * per, s1 = min(nb, s0 + per);
+  TreeSum acc = {0, 0, 0, 0};
+  for (int k = s0; k < s1; ++k) acc = ts_add(acc, tsum[k]);
+  const TreeSum incl = ts_wave_scan(acc, lane);
+  if (lane == 63) wtot[wave] = incl;
+  __syncthreads();
+  TreeSum run = {0, 0, 0, 0};
+  for (int w = 0; w < wave; ++w) run = ts_add(run, wtot[w]);
+  run.a += incl.a - acc.a;
+  run.b += incl.b - acc.b;
+  run.c += incl.c - acc.c;
+  for (int k = s0; k < s1; ++k) {
+    const TreeSum v = tsum[k];
+    tsum[k] = run;
+    run = ts_add(run, v);
+  }
+}
+
+// pre[i] = sum of tri[0..i), pre[n] = the total: block prefix + in-block scan.
+__global__ __launch_bounds__(kBlock) void tree_scan_tiles(int n, const TreeSum* __restrict__ tri,
+                                                          const TreeSum* __restrict__ tsum,
+                                                          TreeSum* __restrict__ pre) {
+  __shared__ TreeSum wtot[kBlock / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  TreeSum v = {0, 0, 0, 0};
+  if (i < n) v = tri[i];
+  const TreeSum incl = ts_wave_scan(v, lane);
+  if (lane == 63) wtot[wave] = incl;
+  __syncthreads();
+  TreeSum run = tsum[blockIdx.x];
+  for (int w = 0; w < wave; ++w) run = ts_add(run, wtot[w]);
+  TreeSum ex = run;
+  ex.a += incl.a - v.a;
+  ex.b += incl.b - v.b;
+  ex.c += incl.c - v.c;
+  if (i < n) pre[i] = ex;
+  if (i == n - 1) pre[n] = ts_add(ex, v);
 }
 
 // Bit j set when entry j of the 16 entries at p (16-aligned) is below d.
@@ -286,7 +400,7 @@ __device__ __forceinline__ int next_lt(const int8_t* __restrict__ L, const MinLe
 
 __global__ __launch_bounds__(kBlock) void bh_emit(
     int n, const uint32_t* __restrict__ vals, const int8_t* __restrict__ L, MinLevels levels,
-    const int* __restrict__ offset, const I64x2* __restrict__ fixpre,
+    const TreeSum* __restrict__ pre,
     BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap) {
   __shared__ unsigned long long masks[kBlock / 64][33];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -299,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
   __syncthreads();
   if (i >= n) return;
   const int D = g->depth;
-  const int M = offset[n];
+  const int M = pre[n].c;
   if (i == 0) {
     g->num_nodes = M;
     if ((int64_t)M > cap) g->overflow = 1;
@@ -307,7 +421,8 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
   const int lp = i > 0 ? (int)L[i - 1] : -1;
   if (lp >= D) return;  // continuation of a merged leaf
   const int internal = max(0, min(li, D - 1) - lp);
-  const int id0 = offset[i];
+  const TreeSum pi = pre[i];
+  const int id0 = pi.c;
   const double r0 = g->root[0], r1 = g->root[1], scale = g->scale;
   const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
   const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
@@ -320,8 +435,9 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
     if ((int64_t)id >= cap) continue;
     const int end = next_lt(L, levels, n, masks, blockIdx.x, i, d);
     const int count = end - i + 1;
-    const long long sx = fixpre[end + 1].a - fixpre[i].a;
-    const long long sy = fixpre[end + 1].b - fixpre[i].b;
+    const TreeSum pe = pre[end + 1];
+    const long long sx = pe.a - pi.a;
+    const long long sy = pe.b - pi.b;
     BhNode nd;
     nd.x = (float)(r0 + ((double)sx / count) / scale);
     nd.y = (float)(r1 + ((double)sy / count) / scale);
@@ -329,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
       nd.z = (int)vals[i];
       nd.w = (int)(0x80000000u | (uint32_t)count);
     } else {
-      nd.z = (end + 1 < n) ? offset[end + 1] : M;
+      nd.z = pe.c;  // first node after the cell (M after the last cell)
       nd.w = (d << 26) | count;
     }
     nodes[id] = nd;
@@ -564,7 +680,6 @@ int bh_alloc(Job& j) {
   b.nb1 = (n + 255) / 256;
   const int64_t c1 = (n + 15) / 16;
   b.hi_stride = ((n + 4095) / 4096 + 15) / 16 * 16 + 16;
-  const int64_t scan_elems = scan_tmp_elems(n + 1);
   int rc;
   fx_ctx* c = j.ctx;
   if ((rc = b.part.alloc(c, sizeof(float4) * kReduceBlocks)) ||
@@ -572,8 +687,9 @@ int bh_alloc(Job& j) {
       (rc = b.vals.alloc(c, 4 * n)) || (rc = b.lcp.alloc(c, pad16(n))) ||
       (rc = b.m1.alloc(c, pad16(c1))) || (rc = b.m2.alloc(c, pad16(b.nb1))) ||
       (rc = b.mhi.alloc(c, 4 * 5 * b.hi_stride)) ||
-      (rc = b.owned.alloc(c, 4 * n)) || (rc = b.offset.alloc(c, 4 * (n + 1))) ||
-      (rc = b.sxp.alloc(c, 16 * n)) || (rc = b.syp.alloc(c, 16 * (n + 1) + 16 * scan_elems)) ||
+      (rc = b.tri.alloc(c, sizeof(TreeSum) * n)) ||
+      (rc = b.tsum.alloc(c, sizeof(TreeSum) * (b.nb1 + 1))) ||
+      (rc = b.pre.alloc(c, sizeof(TreeSum) * (n + 1))) ||
       (rc = b.fx.alloc(c, 4 * n)) || (rc = b.fy.alloc(c, 4 * n)) ||
       (rc = b.stats.alloc(c, 64)) || (rc = b.sort.ensure(c, j.stream, n)) ||
       (rc = fill_i32(j.stream, b.geom.as<int32_t>(), sizeof(BhGeometry) / 4, 0)) ||
@@ -609,7 +725,9 @@ int bh_grow(Job& j, int64_t needed) {
   return FX_OK;
 }
 
-static int bh_build(Job& j) {
+// extra_fill: an int32 array of extra_fill_n entries zeroed by the first
+// kernel (the density grid counts of the same iteration), or null.
+static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n = 0) {
   BhState& b = j.bh;
   const int n = j.N;
   hipStream_t st = j.stream;
@@ -617,12 +735,12 @@ static int bh_build(Job& j) {
   int rc;
   if ((rc = phase_begin(j, kPhaseBhKeys))) return rc;
   hipLaunchKernelGGL(bh_bbox_partial, dim3(kReduceBlocks), dim3(kBlock), 0, st, n,
-                     j.x.as<float>(), j.y.as<float>(), b.part.as<float4>());
-  hipLaunchKernelGGL(bh_geometry, dim3(1), dim3(64), 0, st, b.part.as<float4>(), kReduceBlocks, n, g);
-  FX_HIP(hipGetLastError());
-  if ((rc = fill_i32(st, b.mhi.as<int32_t>(), 5 * b.hi_stride, 127))) return rc;
+                     j.x.as<float>(), j.y.as<float>(), b.part.as<float4>(),
+                     b.mhi.as<int32_t>(), (int64_t)5 * b.hi_stride, 127, extra_fill,
+                     extra_fill ? extra_fill_n : 0, 0);
   hipLaunchKernelGGL(bh_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n, j.x.as<float>(),
-                     j.y.as<float>(), (const BhGeometry*)g, b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
+                     j.y.as<float>(), (const float4*)b.part.as<float4>(), kReduceBlocks, g,
+                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
   FX_HIP(hipGetLastError());
   if ((rc = phase_end(j, kPhaseBhKeys)) || (rc = phase_begin(j, kPhaseBhSort))) return rc;
   if ((rc = radix_sort_pairs(j.ctx, st, b.sort, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), n, 0, 64)))
@@ -631,16 +749,13 @@ static int bh_build(Job& j) {
   hipLaunchKernelGGL(bh_lcp_count, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
                      (const BhGeometry*)g, b.lcp.as<int8_t>(), b.m1.as<int8_t>(), b.m2.as<int8_t>(),
-                     b.mhi.as<int>(), (int)b.hi_stride, b.owned.as<int>(), b.sxp.as<I64x2>());
+                     b.mhi.as<int>(), (int)b.hi_stride, b.tri.as<TreeSum>(), b.tsum.as<TreeSum>());
   FX_HIP(hipGetLastError());
-  const int64_t scan_elems = scan_tmp_elems(n + 1);
-  I64x2* fixpre = b.syp.as<I64x2>();
-  I64x2* tmp = fixpre + (n + 1);
-  // The int32 scan's tile sums share the 16-byte scratch area.
-  if ((rc = scan_exclusive_i32(st, b.owned.as<int32_t>(), b.offset.as<int32_t>(), n,
-                               reinterpret_cast<int32_t*>(tmp), scan_elems)))
-    return rc;
-  if ((rc = scan_exclusive_i64x2(st, b.sxp.p, fixpre, n, tmp, scan_elems))) return rc;
+  hipLaunchKernelGGL(tree_scan_sums, dim3(1), dim3(1024), 0, st, b.tsum.as<TreeSum>(), (int)b.nb1);
+  hipLaunchKernelGGL(tree_scan_tiles, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
+                     (const TreeSum*)b.tri.as<TreeSum>(), (const TreeSum*)b.tsum.as<TreeSum>(),
+                     b.pre.as<TreeSum>());
+  FX_HIP(hipGetLastError());
   MinLevels lv;
   lv.l1 = b.m1.as<int8_t>();
   lv.l2 = b.m2.as<int8_t>();
@@ -648,13 +763,13 @@ static int bh_build(Job& j) {
   lv.hi_stride = (int)b.hi_stride;
   hipLaunchKernelGGL(bh_emit, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
                      b.vals.as<uint32_t>(), b.lcp.as<int8_t>(), lv,
-                     b.offset.as<int>(), (const I64x2*)fixpre, g, b.nodes.as<BhNode>(), b.node_cap);
+                     (const TreeSum*)b.pre.as<TreeSum>(), g, b.nodes.as<BhNode>(), b.node_cap);
   FX_HIP(hipGetLastError());
   return phase_end(j, kPhaseBhTree);
 }
 
-int bh_forces(Job& j) {
-  int rc = bh_build(j);
+int bh_forces(Job& j, int32_t* zero_counts, int64_t zero_counts_n) {
+  int rc = bh_build(j, zero_counts, zero_counts_n);
   if (rc) return rc;
   BhState& b = j.bh;
   if ((rc = phase_begin(j, kPhaseBhWalk))) return rc;

@@ -333,12 +333,13 @@ static void parts3(const DevBuf& a, const DevBuf& b, const DevBuf& c, const uint
 }
 
 // rho -> phi_hat (t1) and E (ex, ey); U = C^T phi_hat is left in t0.
-int dct_field(Job& j) {
+int dct_field(Job& j, bool counts_cleared) {
   DctState& d = j.dct;
   const int G = d.G;
   hipStream_t st = j.stream;
   int rc;
-  if ((rc = fill_i32(st, d.counts.as<int32_t>(), (int64_t)G * G, 0))) return rc;
+  if (!counts_cleared && (rc = fill_i32(st, d.counts.as<int32_t>(), (int64_t)G * G, 0)))
+    return rc;
   const double bin_w = (double)j.die_w / G, bin_h = (double)j.die_h / G;
   if (j.bh.sorted_valid)
     hipLaunchKernelGGL(deposit_sorted_kernel, dim3(div_up(j.N, 256)), dim3(256), 0, st, j.N,

@@ -11,8 +11,10 @@
 //   lv   -= (float)mean
 //   norm  = sqrt(sum(lv^2))            FP64 (174-179)
 //   v     = lv / norm  if norm > 1e-10
-// Three launches per iteration; the reductions use a fixed grid and tree, so
-// results are reproducible run to run.
+// Three launches per iteration. The SpMV runs one row per thread over the
+// whole matrix (every row's sum in the reference's order); its FP64 block
+// partials and those of the centring kernel are reduced by every block in the
+// same fixed order, so results are reproducible run to run.
 
 #include <cmath>
 #include <vector>
@@ -37,11 +39,12 @@ __device__ __forceinline__ double block_sum_d(double v) {
   return r;
 }
 
-// Every block re-reduces the kGrid partials in the same order, so all see
-// the same total.
-__device__ __forceinline__ double total_of(const double* part) {
+// Every block re-reduces the `count` partials in the same order (thread t
+// sums t, t + kBlock, ...; then a fixed tree), so all see the same total.
+__device__ __forceinline__ double total_of(const double* part, int count = kGrid) {
   __shared__ double tot;
-  double v = (threadIdx.x < kGrid) ? part[threadIdx.x] : 0.0;
+  double v = 0.0;
+  for (int k = threadIdx.x; k < count; k += kBlock) v += part[k];
   const double r = block_sum_d(v);
   if (threadIdx.x == 0) tot = r;
   __syncthreads();
@@ -54,20 +57,22 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int* __restri
                                                       const float* __restrict__ v,
                                                       float* __restrict__ lv, double* part) {
   double s = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
     float a = deg[i] * v[i];
     const int e = rp[i + 1];
     for (int k = rp[i]; k < e; ++k) a -= v[col[k]];
     lv[i] = a;
-    s += (double)a;
+    s = (double)a;
   }
   const double r = block_sum_d(s);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
 __global__ __launch_bounds__(kBlock) void center_kernel(int n, float* __restrict__ lv,
-                                                        const double* part_sum, double* part_sq) {
-  const float mean = (float)(total_of(part_sum) / (double)n);
+                                                        const double* part_sum, int nsum,
+                                                        double* part_sq) {
+  const float mean = (float)(total_of(part_sum, nsum) / (double)n);
   double s = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float t = lv[i] - mean;
@@ -164,7 +169,8 @@ extern "C" int fx_fiedler(fx_ctx* ctx, int32_t n, const int32_t* gates, int32_t 
   int rc;
   if ((rc = drp.alloc(ctx, 4 * (size_t)(n + 1))) || (rc = dcol.alloc(ctx, 4 * (col.size() + 1))) ||
       (rc = ddeg.alloc(ctx, 4 * (size_t)n)) || (rc = dv.alloc(ctx, 4 * (size_t)n)) ||
-      (rc = dlv.alloc(ctx, 4 * (size_t)n)) || (rc = dpart.alloc(ctx, 8 * 3 * fx::kGrid + 64)))
+      (rc = dlv.alloc(ctx, 4 * (size_t)n)) ||
+      (rc = dpart.alloc(ctx, 8 * (3 * (size_t)fx::kGrid + fx::div_up(n, fx::kBlock)) + 64)))
     return rc;
   FX_HIP(hipMemcpyAsync(drp.p, rp.data(), 4 * (size_t)(n + 1), hipMemcpyHostToDevice, st));
   if (!col.empty())
@@ -175,6 +181,8 @@ extern "C" int fx_fiedler(fx_ctx* ctx, int32_t n, const int32_t* gates, int32_t 
   double* psq = psum + fx::kGrid;
   double* pden = psq + fx::kGrid;
   double* out = pden + fx::kGrid;
+  double* prow = out + 8;  // SpMV block partials, one per row block
+  const int nrb = (int)fx::div_up(n, fx::kBlock);
   const dim3 grid(fx::kGrid), block(fx::kBlock);
   // Initial normalisation (partitioner.rs:149-155).
   hipLaunchKernelGGL(fx::sumsq_kernel, grid, block, 0, st, n, dlv.as<float>(), psq);
@@ -182,9 +190,9 @@ extern "C" int fx_fiedler(fx_ctx* ctx, int32_t n, const int32_t* gates, int32_t 
   FX_HIP(hipGetLastError());
   FX_HIP(hipEventRecord(ctx->ev0, st));
   for (int it = 0; it < iterations; ++it) {
-    hipLaunchKernelGGL(fx::spmv_kernel, grid, block, 0, st, n, drp.as<int>(), dcol.as<int>(),
-                       ddeg.as<float>(), dv.as<float>(), dlv.as<float>(), psum);
-    hipLaunchKernelGGL(fx::center_kernel, grid, block, 0, st, n, dlv.as<float>(), psum, psq);
+    hipLaunchKernelGGL(fx::spmv_kernel, dim3((unsigned)nrb), block, 0, st, n, drp.as<int>(),
+                       dcol.as<int>(), ddeg.as<float>(), dv.as<float>(), dlv.as<float>(), prow);
+    hipLaunchKernelGGL(fx::center_kernel, grid, block, 0, st, n, dlv.as<float>(), prow, nrb, psq);
     hipLaunchKernelGGL(fx::normalise_kernel, grid, block, 0, st, n, dlv.as<float>(), psq, dv.as<float>());
   }
   FX_HIP(hipGetLastError());

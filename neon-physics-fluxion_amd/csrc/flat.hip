@@ -449,9 +449,15 @@ static int enqueue_phases(Job& j, int comps, bool integrate) {
         (rc = phase_end(j, kPhaseSprings)))
       return rc;
   }
-  if ((comps & 2) && (rc = bh_forces(j))) return rc;
+  // With both, the tree build's first kernel also clears the density counts.
+  const bool shared_fill = (comps & 2) && (comps & 4);
+  if (comps & 2) {
+    const int64_t G = j.p.density_grid;
+    if ((rc = bh_forces(j, shared_fill ? j.dct.counts.as<int32_t>() : nullptr, G * G))) return rc;
+  }
   if (comps & 4) {
-    if ((rc = phase_begin(j, kPhaseDct)) || (rc = dct_field(j)) || (rc = phase_end(j, kPhaseDct)))
+    if ((rc = phase_begin(j, kPhaseDct)) || (rc = dct_field(j, shared_fill)) ||
+        (rc = phase_end(j, kPhaseDct)))
       return rc;
   }
   if ((rc = phase_begin(j, kPhaseCombine)) || (rc = combine(j, comps, integrate)) ||

@@ -33,7 +33,7 @@ struct BhState {
   // lcp: L[i] = common levels of keys i and i+1 (int8, -1 at n-1); m1, m2: minima of L over
   // 16 and 256 positions (int8); mhi: minima over 16^3 .. 16^7 positions (int32, one level
   // every hi_stride entries). Padding entries read 127 so searches need no bounds.
-  DevBuf part, geom, keys, vals, lcp, m1, m2, mhi, owned, offset, sxp, syp, nodes, fx, fy,
+  DevBuf part, geom, keys, vals, lcp, m1, m2, mhi, tri, tsum, pre, nodes, fx, fy,
       stats;
   SortWorkspace sort;
   bool sorted_valid = false;  // vals holds a permutation from a completed build
@@ -102,8 +102,12 @@ int phase_end(Job& j, int phase);
 
 // Phases (each enqueues on job.stream; no host synchronisation).
 int springs_centroids(Job& j);
-int bh_forces(Job& j);             // writes j.bh.fx/fy (per original index)
-int dct_field(Job& j);             // writes j.dct.phi
+// Writes j.bh.fx/fy (per original index). zero_counts (may be null): an int32
+// array of zero_counts_n entries cleared by the build's first kernel.
+int bh_forces(Job& j, int32_t* zero_counts = nullptr, int64_t zero_counts_n = 0);
+// Writes the field; counts_cleared: the point counts were already zeroed
+// earlier in the iteration (by bh_forces).
+int dct_field(Job& j, bool counts_cleared = false);
 int combine(Job& j, int components, bool integrate);  // forces -> (fxo,fyo) or integrate
 int bh_alloc(Job& j);
 int dct_alloc(Job& j);

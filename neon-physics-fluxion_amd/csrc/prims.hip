@@ -2,17 +2,19 @@
 // pairs and exclusive prefix scans (int32, int64 pairs).
 //
 // Radix sort (8-bit digits, "onesweep" structure): one kernel counts the
-// digits of all passes at once and one small kernel turns them into global
-// digit bases; then each pass is ONE kernel over tiles of 1024 items. A tile
-// takes the next tile id from an atomic counter (ids follow start order, so a
-// tile only ever waits on tiles that are already running), counts its
-// digits, publishes them, and sums its predecessors' counts per digit by
+// digits of all passes at once; then each pass is ONE kernel over tiles of
+// 1024 items. A tile takes the next tile id from an atomic counter (ids
+// follow start order, so a tile only ever waits on tiles that are already
+// running), scans the pass's 256 global digit counts into digit bases,
+// counts its own digits, publishes them, and sums its predecessors' counts per digit by
 // decoupled look-back: a predecessor's status word holds either its own
 // count (aggregate) or its inclusive prefix (which ends the walk back). The
 // tile then ranks its items stably — rounds of 256 in input order; inside a
 // 64-lane wave, eight ballots match lanes with equal digits — and scatters
 // them. Status words of the next pass are cleared by the current one
-// (double buffering), so nothing needs a memset inside a captured graph.
+// (double buffering), and the sort's last kernel clears the digit counts
+// and tile counters for the next sort, so nothing needs a memset inside a
+// captured graph.
 // The look-back wait is bounded in time (wall clock): on timeout a flag is
 // raised, no item of a timed-out tile is written, every scatter index is
 // checked against n, and a last kernel replaces the output by the identity
@@ -30,13 +32,8 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kStAgg = 1u << 30, kStInc = 2u << 30, kStMask = (1u << 30) - 1;
 constexpr double kLookBackSeconds = 0.2;
 
-// Zeroes the 8x256 global digit counts and the per-pass tile counters.
-__global__ void radix_reset(uint32_t* __restrict__ ghist, uint32_t* __restrict__ counters) {
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) ghist[i] = 0;
-  if (threadIdx.x < 8) counters[threadIdx.x] = 0;
-}
-
-// Digit counts of every pass in one read of the keys.
+// Digit counts of every pass in one read of the keys (added to counts that
+// the previous sort's last kernel, or the workspace set-up, left at zero).
 __global__ __launch_bounds__(256) void radix_histogram(const uint64_t* __restrict__ keys,
                                                        int64_t n, int lo_bit, int passes,
                                                        uint32_t* __restrict__ ghist) {
@@ -53,36 +50,16 @@ __global__ __launch_bounds__(256) void radix_histogram(const uint64_t* __restric
     if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], h[p][threadIdx.x]);
 }
 
-// Exclusive prefix of each pass's 256 digit counts (one wave per pass).
-__global__ void radix_bases(uint32_t* __restrict__ ghist, int passes) {
-  const int p = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (p >= passes) return;
-  uint32_t v[4], s = 0;
-  for (int k = 0; k < 4; ++k) {
-    v[k] = ghist[p * 256 + lane * 4 + k];
-    s += v[k];
-  }
-  uint32_t incl = s;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t u = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += u;
-  }
-  uint32_t run = incl - s;
-  for (int k = 0; k < 4; ++k) {
-    ghist[p * 256 + lane * 4 + k] = run;
-    run += v[k];
-  }
-}
-
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
-    int ntiles, const uint32_t* __restrict__ gbase, uint32_t* __restrict__ status,
+    int ntiles, const uint32_t* __restrict__ gcount, uint32_t* __restrict__ status,
     uint32_t* __restrict__ status_next, uint32_t* __restrict__ counter,
     uint32_t* __restrict__ fault, uint64_t spin_ticks, uint64_t* __restrict__ kout,
     uint32_t* __restrict__ vout) {
   __shared__ uint32_t cnt[256];
   __shared__ uint32_t base[256];
   __shared__ uint32_t wcount[4][256];
+  __shared__ uint32_t wsum[4];
   __shared__ int tile_s;
   __shared__ int timed_out;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -91,7 +68,17 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     timed_out = 0;
   }
   cnt[t] = 0;
+  // Global base of digit t: exclusive prefix of this pass's digit counts.
+  const uint32_t c_t = gcount[t];
+  uint32_t incl = c_t;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += u;
+  }
+  if (lane == 63) wsum[wave] = incl;
   __syncthreads();
+  uint32_t gb = incl - c_t;
+  for (int w = 0; w < wave; ++w) gb += wsum[w];
   const int tile = tile_s;
   if (tile >= ntiles) {  // more ids than workgroups: the counter was not reset
     if (t == 0) atomicOr(fault, 2u);
@@ -118,7 +105,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
   uint32_t* st = status + (int64_t)tile * 256 + t;
   if (tile == 0) {
     __hip_atomic_store(st, kStInc | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    base[t] = gbase[t];
+    base[t] = gb;
   } else {
     __hip_atomic_store(st, kStAgg | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t excl = 0;
@@ -141,7 +128,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
       --b;
     }
     __hip_atomic_store(st, kStInc | (excl + mine), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    base[t] = gbase[t] + excl;
+    base[t] = gb + excl;
   }
   __syncthreads();
   const bool write_ok = timed_out == 0;
@@ -178,10 +165,17 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
   }
 }
 
-// After a failed sort (fault flag set): identity permutation and zero keys,
-// so that consumers of the permutation index in range.
-__global__ void radix_repair(const uint32_t* __restrict__ fault, uint64_t* __restrict__ keys,
+// Last kernel of a sort: clears the digit counts and tile counters for the
+// next sort; after a failed sort (fault flag set) writes the identity
+// permutation and zero keys, so that consumers of the permutation index in
+// range.
+__global__ void radix_repair(const uint32_t* __restrict__ fault, uint32_t* __restrict__ ghist,
+                             uint32_t* __restrict__ counters, uint64_t* __restrict__ keys,
                              uint32_t* __restrict__ vals, int64_t n) {
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) ghist[i] = 0;
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
+  }
   if (*fault == 0) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -231,10 +225,10 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
   uint32_t* va = vals;
   uint64_t* kb = ws.keys_alt.as<uint64_t>();
   uint32_t* vb = ws.vals_alt.as<uint32_t>();
-  hipLaunchKernelGGL(radix_reset, dim3(1), dim3(256), 0, st, gbase, counters);
-  const unsigned hblocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  // Few blocks, each counting many keys: the per-block flush of 8 x 256
+  // counts is the costly part.
+  const unsigned hblocks = (unsigned)std::min<int64_t>((n + 2047) / 2048, 256);
   hipLaunchKernelGGL(radix_histogram, dim3(hblocks), dim3(256), 0, st, ka, n, lo_bit, passes, gbase);
-  hipLaunchKernelGGL(radix_bases, dim3(1), dim3(512), 0, st, gbase, passes);
   for (int p = 0; p < passes; ++p) {
     uint32_t* cur = (p & 1) ? status1 : status0;
     uint32_t* nxt = (p & 1) ? status0 : status1;
@@ -254,7 +248,7 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
     FX_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
   }
   hipLaunchKernelGGL(radix_repair, dim3((unsigned)std::min<int64_t>(div_up(n, 256), 1024)),
-                     dim3(256), 0, st, (const uint32_t*)fault, keys, vals, n);
+                     dim3(256), 0, st, (const uint32_t*)fault, gbase, counters, keys, vals, n);
   FX_HIP(hipGetLastError());
   return FX_OK;
 }
