@@ -203,17 +203,28 @@ def leaves_workload(fxn, ctx, num_leaves=4096, iters=300):
            "algorithmic_flop": flops}
     for mode, fast in (("exact", False), ("fast", True)):
         pl = fxn.Placement.new(circ.num_gates)
-        fxn.place_leaves(ctx, leaf_ptr, gates, reg, circ, pl, cfg, fast=fast)  # warm-up
-        best, wall = 1e30, 1e30
-        for _ in range(3):
-            t0 = time.perf_counter()
-            ms = fxn.place_leaves(ctx, leaf_ptr, gates, reg, circ, pl, cfg, fast=fast)
-            wall = min(wall, (time.perf_counter() - t0) * 1e3)
-            best = min(best, ms)
-        ach = flops / (best / 1e3) / 1e12
-        out[mode] = {"kernel_ms": best, "call_ms": wall, "achieved_tflops": ach,
-                     "frac_fp32_peak": ach / PEAK_FP32_TFLOPS,
-                     "leaf_iters_per_s": num_leaves * iters / (best / 1e3)}
+        default_tile = fxn.leaves_tile(ctx, fast)
+        res = {}
+        for label in ("default_tile", "autotuned"):
+            if label == "autotuned":
+                tile, _ = fxn.autotune_leaves(ctx, leaf_ptr, gates, reg, circ, cfg, iterations=20,
+                                              fast=fast)
+            tile = fxn.leaves_tile(ctx, fast)
+            fxn.place_leaves(ctx, leaf_ptr, gates, reg, circ, pl, cfg, fast=fast)  # warm-up
+            best, wall = 1e30, 1e30
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ms = fxn.place_leaves(ctx, leaf_ptr, gates, reg, circ, pl, cfg, fast=fast)
+                wall = min(wall, (time.perf_counter() - t0) * 1e3)
+                best = min(best, ms)
+            ach = flops / (best / 1e3) / 1e12
+            res[label] = {"tile": {"gates_per_lane": tile[0], "min_lanes": tile[1],
+                                   "block": tile[2]},
+                          "kernel_ms": best, "call_ms": wall, "achieved_tflops": ach,
+                          "frac_fp32_peak": ach / PEAK_FP32_TFLOPS,
+                          "leaf_iters_per_s": num_leaves * iters / (best / 1e3)}
+        fxn.set_leaves_tile(ctx, default_tile, fast)
+        out[mode] = res
     return out
 
 

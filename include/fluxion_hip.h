@@ -121,6 +121,25 @@ int fx_place_leaves_state(fx_ctx* ctx, const fx_leaf_batch* batch, const fx_para
 /* Milliseconds of the last fx_place_leaves* kernel (HIP events). */
 double fx_leaves_last_kernel_ms(const fx_ctx* ctx);
 
+/* Tile shape of the leaf launch (no counterpart in the reference, whose leaf
+ * loop is sequential, placer.rs:66-69): each leaf of n gates gets a segment
+ * of lanes (8..256) with at most gates_per_lane gates per lane and at least
+ * min_lanes lanes; segments are packed into workgroups of `block` threads.
+ * Exact-mode results do not depend on the shape. Default {1, 64, 256}. */
+typedef struct fx_leaf_tile {
+  int32_t gates_per_lane; /* 1, 2 or 4 */
+  int32_t min_lanes;      /* 8, 16, 32 or 64 */
+  int32_t block;          /* 64, 128 or 256 */
+} fx_leaf_tile;
+/* The shape used for `mode` (FX_LEAF_EXACT / FX_LEAF_FAST) on this context. */
+int fx_leaves_set_tile(fx_ctx* ctx, int32_t mode, const fx_leaf_tile* tile);
+int fx_leaves_get_tile(fx_ctx* ctx, int32_t mode, fx_leaf_tile* tile);
+/* Autotuner: runs every valid shape on this batch for `iterations`
+ * iterations from the initial positions (twice; the second run is timed),
+ * keeps the fastest for `mode` on the context and returns it with its time. */
+int fx_leaves_autotune(fx_ctx* ctx, const fx_leaf_batch* batch, const fx_params* params,
+                       int32_t mode, int32_t iterations, fx_leaf_tile* best, double* best_ms);
+
 /* ---- flat global placement job (device-resident state) ----
  * The dispatch_forces operator (HAPR_fluxion_v4/src/gpu/dispatch.rs:18-106)
  * reworked so that the netlist, positions, velocities, Barnes-Hut tree and

@@ -2,22 +2,27 @@
 // (HAPR_fluxion_v4/src/algorithm/placer.rs:66-69) running place_gates_in_leaf
 // (placer.rs:210-287) for every leaf in ONE kernel launch.
 //
-// Layout: leaves are bin-packed into 256-thread workgroups by whole 64-lane
-// waves (first-fit decreasing on ceil(n/64) waves, a leaf above 256 gates
-// takes a whole workgroup and each thread owns up to 4 gates). Bins are
-// dispatched longest first. A workgroup keeps its leaves' positions in LDS
-// for all iterations; velocities live in registers; each thread owns gates
-// and evaluates, per iteration, the in-leaf adjacency springs and the exact
-// repulsion against every other gate of its leaf, reading positions by LDS
-// broadcast (all lanes of a wave read the same address).
+// Layout (the "tile shape"): every leaf gets a segment of S lanes, S one of
+// 8, 16, 32, 64, 128, 192, 256 — the smallest one giving at most
+// `gates_per_lane` gates per lane, at least `min_lanes`, at most the
+// workgroup size — and each lane owns gates i, i+S, ... (at most 4). Segments
+// are packed into workgroups of `block` threads (largest first, next fit;
+// segments of up to 64 lanes never straddle a wave) with at most 1024 gates
+// per workgroup; workgroups are dispatched longest first. A workgroup keeps
+// its leaves' positions in LDS for all iterations; velocities live in
+// registers; each lane evaluates, per iteration, its gates' in-leaf adjacency
+// springs and the exact repulsion against every other gate of the leaf,
+// reading positions from LDS. The shape is picked per context and mode by
+// fx_leaves_autotune, which times every candidate on the caller's batch.
 //
 // Exact mode reproduces the reference's FP32 arithmetic bit for bit: the
 // reference accumulates, for gate i, its springs in adjacency order, then
 // -f(k,i) for k<i (rows before i) and +f(i,j) for j>i; since
 // f(k,i) = -f(i,k) exactly in IEEE arithmetic, the per-gate sum over j in
-// ascending order of f(i,j) is the same sequence of roundings. Division and
-// square root are IEEE (correctly rounded), and the build disables FMA
-// contraction. Fast mode uses v_rsq_f32 and FMA (within 1e-4 of exact).
+// ascending order of f(i,j) is the same sequence of roundings — for every
+// tile shape. Division and square root are IEEE (correctly rounded), and the
+// build disables FMA contraction. Fast mode uses v_rsq_f32 and FMA (within
+// 1e-4 of exact).
 
 #include <algorithm>
 #include <cstring>
@@ -27,35 +32,34 @@
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kMaxBlock = 256;
 constexpr int kMaxLeaf = 1024;
-constexpr int kMaxPerThread = kMaxLeaf / kThreads;
+constexpr int kMaxPerThread = 4;
 
+// Segment descriptor: (leaf, first lane in the workgroup, lanes, LDS offset).
 template <bool kFast>
-__global__ __launch_bounds__(kThreads) void place_leaves_kernel(
-    const int4* __restrict__ bins, const int* __restrict__ leaf_base,
-    const int* __restrict__ leaf_n, const float4* __restrict__ region,
-    const int* __restrict__ lptr, const int* __restrict__ lnb, float attract, float repel,
-    float damping, float dt, int iters, int init, float* __restrict__ px,
-    float* __restrict__ py, float* __restrict__ pvx, float* __restrict__ pvy,
-    float* __restrict__ pfx, float* __restrict__ pfy) {
+__global__ __launch_bounds__(kMaxBlock) void place_leaves_kernel(
+    const int* __restrict__ wg_ptr, const int4* __restrict__ segs,
+    const int* __restrict__ leaf_base, const int* __restrict__ leaf_n,
+    const float4* __restrict__ region, const int* __restrict__ lptr,
+    const int* __restrict__ lnb, float attract, float repel, float damping, float dt, int iters,
+    int init, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pvx,
+    float* __restrict__ pvy, float* __restrict__ pfx, float* __restrict__ pfy) {
   __shared__ float sx[kMaxLeaf];
   __shared__ float sy[kMaxLeaf];
-  const int4 bin = bins[blockIdx.x];
-  const int slots[4] = {bin.x, bin.y, bin.z, bin.w};
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int leaf = slots[wave];
-  int first = wave, count = 0;
-  if (leaf >= 0) {
-    while (first > 0 && slots[first - 1] == leaf) --first;
-    for (int s = first; s < 4 && slots[s] == leaf; ++s) ++count;
+  const int tid = threadIdx.x;
+  int leaf = -1, t = 0, T = 1, lo = 0;
+  for (int s = wg_ptr[blockIdx.x]; s < wg_ptr[blockIdx.x + 1]; ++s) {
+    const int4 d = segs[s];
+    if (tid >= d.y && tid < d.y + d.z) {
+      leaf = d.x;
+      t = tid - d.y;
+      T = d.z;
+      lo = d.w;
+    }
   }
   const int n = leaf >= 0 ? leaf_n[leaf] : 0;
   const int base = leaf >= 0 ? leaf_base[leaf] : 0;
-  const int lo = first * 64;  // LDS offset of this leaf
-  const int t = (wave - first) * 64 + lane;
-  const int T = count * 64;
   float4 reg = leaf >= 0 ? region[leaf] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float half_w = reg.z / 2.0f, half_h = reg.w / 2.0f;
   const float min_x = reg.x - half_w, min_y = reg.y - half_h;
@@ -162,25 +166,79 @@ __global__ __launch_bounds__(kThreads) void place_leaves_kernel(
   }
 }
 
-struct Bin {
-  int slot[4] = {-1, -1, -1, -1};
-  int used = 0;
-  double cost = 0.0;
+bool tile_ok(const fx_leaf_tile& t) {
+  return (t.gates_per_lane == 1 || t.gates_per_lane == 2 || t.gates_per_lane == 4) &&
+         (t.min_lanes == 8 || t.min_lanes == 16 || t.min_lanes == 32 || t.min_lanes == 64) &&
+         (t.block == 64 || t.block == 128 || t.block == 256);
+}
+
+// Lanes of a leaf of n gates under tile shape t (0: the shape cannot hold it).
+int lanes_for(int n, const fx_leaf_tile& t) {
+  static const int kSizes[] = {8, 16, 32, 64, 128, 192, 256};
+  const int want = std::max(t.min_lanes, (n + t.gates_per_lane - 1) / t.gates_per_lane);
+  int S = 256;
+  for (int c : kSizes)
+    if (c >= want) {
+      S = c;
+      break;
+    }
+  S = std::min(S, t.block);
+  return (n + S - 1) / S <= kMaxPerThread ? S : 0;
+}
+
+// Packs the leaves into workgroups; returns false when a leaf does not fit.
+bool build_tiles(const std::vector<int>& leaf_n, const fx_leaf_tile& t, std::vector<int>& wg_ptr,
+                 std::vector<int4>& segs) {
+  const int L = (int)leaf_n.size();
+  std::vector<int> S(L, 0), order;
+  for (int c = 0; c < L; ++c) {
+    if (leaf_n[c] == 0) continue;
+    S[c] = lanes_for(leaf_n[c], t);
+    if (!S[c]) return false;
+    order.push_back(c);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    return S[a] != S[c] ? S[a] > S[c] : leaf_n[a] > leaf_n[c];
+  });
+  struct Wg {
+    std::vector<int4> segs;
+    double cost = 0.0;
+  };
+  std::vector<Wg> wgs;
+  int lane = t.block, lds = 0;
+  for (int c : order) {
+    const int n = leaf_n[c];
+    if (lane + S[c] > t.block || lds + n > kMaxLeaf) {
+      wgs.emplace_back();
+      lane = 0;
+      lds = 0;
+    }
+    Wg& w = wgs.back();
+    w.segs.push_back(make_int4(c, lane, S[c], lds));
+    w.cost = std::max(w.cost, (double)n * (double)((n + S[c] - 1) / S[c]));
+    lane += S[c];
+    lds += n;
+  }
+  std::stable_sort(wgs.begin(), wgs.end(), [](const Wg& a, const Wg& c) { return a.cost > c.cost; });
+  wg_ptr.assign(1, 0);
+  segs.clear();
+  for (const Wg& w : wgs) {
+    segs.insert(segs.end(), w.segs.begin(), w.segs.end());
+    wg_ptr.push_back((int)segs.size());
+  }
+  return true;
+}
+
+// A validated batch uploaded to the device (leaf-local layout).
+struct LeafBatch {
+  int L = 0, P = 0;
+  std::vector<int> leaf_n;
+  fx::DevBuf dbase, dn, dreg, dlptr, dlnb, dx, dy, dvx, dvy, dfx, dfy, dwg, dsegs;
 };
 
-}  // namespace
-
-extern "C" int fx_place_leaves_state(fx_ctx* ctx, const fx_leaf_batch* b, const fx_params* p,
-                                     int32_t mode, int32_t init, int32_t iterations, float* x,
-                                     float* y, float* vx, float* vy, float* fx, float* fy) {
-  fx::clear_error();
-  FX_CHECK(ctx && b && p && x && y, FX_E_INVALID, "fx_place_leaves: NULL argument");
-  FX_CHECK(mode == FX_LEAF_EXACT || mode == FX_LEAF_FAST, FX_E_INVALID, "bad mode %d", mode);
-  FX_CHECK(iterations >= 0, FX_E_INVALID, "negative iteration count");
-  FX_CHECK(b->num_leaves >= 0 && b->num_gates >= 0, FX_E_INVALID, "negative sizes");
-  FX_CHECK(!init || true, FX_E_INVALID, "");
+int prepare(fx_ctx* ctx, const fx_leaf_batch* b, int init, const float* x, const float* y,
+            const float* vx, const float* vy, bool want_forces, LeafBatch& lb) {
   const int L = b->num_leaves;
-  if (L == 0) return FX_OK;
   FX_CHECK(b->leaf_ptr && b->leaf_gates && b->region && b->adj_ptr && b->adj, FX_E_INVALID,
            "fx_place_leaves: NULL batch array");
   FX_CHECK(b->leaf_ptr[0] == 0, FX_E_INVALID, "leaf_ptr[0] must be 0");
@@ -214,38 +272,6 @@ extern "C" int fx_place_leaves_state(fx_ctx* ctx, const fx_leaf_batch* b, const 
     }
     for (int i = s; i < e; ++i) g2l[b->leaf_gates[i]] = -1;
   }
-  // Bin packing by waves (first-fit decreasing), then longest bins first.
-  std::vector<int> order(L);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return leaf_n[a] > leaf_n[c]; });
-  std::vector<Bin> bins;
-  std::vector<int> open[4];  // open bins indexed by free wave slots - 1
-  for (int c : order) {
-    const int n = leaf_n[c];
-    if (n == 0) continue;
-    const int w = std::min(4, (n + 63) / 64);
-    const double cost = (double)n * (double)((n + 64 * w - 1) / (64 * w));
-    int bi = -1;
-    for (int f = w; f <= 4 && bi < 0; ++f) {
-      if (!open[f - 1].empty()) {
-        bi = open[f - 1].back();
-        open[f - 1].pop_back();
-      }
-    }
-    if (bi < 0) {
-      bins.emplace_back();
-      bi = (int)bins.size() - 1;
-    }
-    Bin& bn = bins[bi];
-    for (int s = 0; s < w; ++s) bn.slot[bn.used + s] = c;
-    bn.used += w;
-    bn.cost = std::max(bn.cost, cost);
-    if (bn.used < 4) open[4 - bn.used - 1].push_back(bi);
-  }
-  std::stable_sort(bins.begin(), bins.end(), [](const Bin& a, const Bin& c) { return a.cost > c.cost; });
-  std::vector<int4> hbins(bins.size());
-  for (size_t i = 0; i < bins.size(); ++i)
-    hbins[i] = make_int4(bins[i].slot[0], bins[i].slot[1], bins[i].slot[2], bins[i].slot[3]);
   std::vector<float4> hreg(L);
   for (int c = 0; c < L; ++c)
     hreg[c] = make_float4(b->region[4 * c], b->region[4 * c + 1], b->region[4 * c + 2],
@@ -261,63 +287,122 @@ extern "C" int fx_place_leaves_state(fx_ctx* ctx, const fx_leaf_batch* b, const 
         hvy[i] = vy[g];
       }
     }
-  if (bins.empty()) return FX_OK;
-
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  FX_HIP(hipSetDevice(ctx->device));
-  fx::DevBuf dbins, dbase, dn, dreg, dlptr, dlnb, dx, dy, dvx, dvy, dfx, dfy;
+  lb.L = L;
+  lb.P = P;
+  lb.leaf_n = leaf_n;
   int rc;
-  if ((rc = dbins.alloc(ctx, sizeof(int4) * hbins.size())) || (rc = dbase.alloc(ctx, 4 * L)) ||
-      (rc = dn.alloc(ctx, 4 * L)) || (rc = dreg.alloc(ctx, sizeof(float4) * L)) ||
-      (rc = dlptr.alloc(ctx, 4 * (P + 1))) || (rc = dlnb.alloc(ctx, 4 * (lnb.size() + 1))) ||
-      (rc = dx.alloc(ctx, 4 * P)) || (rc = dy.alloc(ctx, 4 * P)) || (rc = dvx.alloc(ctx, 4 * P)) ||
-      (rc = dvy.alloc(ctx, 4 * P)))
+  if ((rc = lb.dbase.alloc(ctx, 4 * L)) || (rc = lb.dn.alloc(ctx, 4 * L)) ||
+      (rc = lb.dreg.alloc(ctx, sizeof(float4) * L)) || (rc = lb.dlptr.alloc(ctx, 4 * (P + 1))) ||
+      (rc = lb.dlnb.alloc(ctx, 4 * (lnb.size() + 1))) || (rc = lb.dx.alloc(ctx, 4 * P + 4)) ||
+      (rc = lb.dy.alloc(ctx, 4 * P + 4)) || (rc = lb.dvx.alloc(ctx, 4 * P + 4)) ||
+      (rc = lb.dvy.alloc(ctx, 4 * P + 4)))
     return rc;
-  if (fx && fy) {
-    if ((rc = dfx.alloc(ctx, 4 * P)) || (rc = dfy.alloc(ctx, 4 * P))) return rc;
+  if (want_forces) {
+    if ((rc = lb.dfx.alloc(ctx, 4 * P + 4)) || (rc = lb.dfy.alloc(ctx, 4 * P + 4))) return rc;
   }
   hipStream_t st = ctx->stream;
-  FX_HIP(hipMemcpyAsync(dbins.p, hbins.data(), sizeof(int4) * hbins.size(), hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dbase.p, leaf_base.data(), 4 * L, hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dn.p, leaf_n.data(), 4 * L, hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dreg.p, hreg.data(), sizeof(float4) * L, hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dlptr.p, lptr.data(), 4 * (P + 1), hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(lb.dbase.p, leaf_base.data(), 4 * L, hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(lb.dn.p, leaf_n.data(), 4 * L, hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(lb.dreg.p, hreg.data(), sizeof(float4) * L, hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(lb.dlptr.p, lptr.data(), 4 * (P + 1), hipMemcpyHostToDevice, st));
   if (!lnb.empty())
-    FX_HIP(hipMemcpyAsync(dlnb.p, lnb.data(), 4 * lnb.size(), hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dx.p, hx.data(), 4 * P, hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dy.p, hy.data(), 4 * P, hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dvx.p, hvx.data(), 4 * P, hipMemcpyHostToDevice, st));
-  FX_HIP(hipMemcpyAsync(dvy.p, hvy.data(), 4 * P, hipMemcpyHostToDevice, st));
-  const dim3 grid((unsigned)hbins.size()), block(kThreads);
+    FX_HIP(hipMemcpyAsync(lb.dlnb.p, lnb.data(), 4 * lnb.size(), hipMemcpyHostToDevice, st));
+  if (P) {
+    FX_HIP(hipMemcpyAsync(lb.dx.p, hx.data(), 4 * P, hipMemcpyHostToDevice, st));
+    FX_HIP(hipMemcpyAsync(lb.dy.p, hy.data(), 4 * P, hipMemcpyHostToDevice, st));
+    FX_HIP(hipMemcpyAsync(lb.dvx.p, hvx.data(), 4 * P, hipMemcpyHostToDevice, st));
+    FX_HIP(hipMemcpyAsync(lb.dvy.p, hvy.data(), 4 * P, hipMemcpyHostToDevice, st));
+  }
+  FX_HIP(hipStreamSynchronize(st));  // the host vectors die with this frame
+  return FX_OK;
+}
+
+// Uploads the tiling of shape t and launches; the launch is bracketed by the
+// context's events. ntiles: number of workgroups (0: nothing to do).
+int launch(fx_ctx* ctx, LeafBatch& lb, const fx_leaf_tile& t, const fx_params* p, int32_t mode,
+           int init, int iterations, int* ntiles) {
+  std::vector<int> wg_ptr;
+  std::vector<int4> segs;
+  FX_CHECK(build_tiles(lb.leaf_n, t, wg_ptr, segs), FX_E_INVALID,
+           "tile shape (%d gates/lane, %d min lanes, block %d) cannot hold a leaf of this batch",
+           t.gates_per_lane, t.min_lanes, t.block);
+  *ntiles = (int)wg_ptr.size() - 1;
+  if (*ntiles <= 0) return FX_OK;
+  int rc;
+  if ((rc = lb.dwg.alloc(ctx, 4 * wg_ptr.size())) || (rc = lb.dsegs.alloc(ctx, sizeof(int4) * segs.size())))
+    return rc;
+  hipStream_t st = ctx->stream;
+  FX_HIP(hipMemcpyAsync(lb.dwg.p, wg_ptr.data(), 4 * wg_ptr.size(), hipMemcpyHostToDevice, st));
+  FX_HIP(hipMemcpyAsync(lb.dsegs.p, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice, st));
+  const dim3 grid((unsigned)*ntiles), block((unsigned)t.block);
   FX_HIP(hipEventRecord(ctx->ev0, st));
   if (mode == FX_LEAF_FAST)
-    hipLaunchKernelGGL(place_leaves_kernel<true>, grid, block, 0, st, dbins.as<int4>(),
-                       dbase.as<int>(), dn.as<int>(), dreg.as<float4>(), dlptr.as<int>(),
-                       dlnb.as<int>(), p->attract_weight, p->repel_weight, p->damping, p->dt,
-                       iterations, init ? 1 : 0, dx.as<float>(), dy.as<float>(), dvx.as<float>(),
-                       dvy.as<float>(), dfx.as<float>(), dfy.as<float>());
+    hipLaunchKernelGGL(place_leaves_kernel<true>, grid, block, 0, st, lb.dwg.as<int>(),
+                       lb.dsegs.as<int4>(), lb.dbase.as<int>(), lb.dn.as<int>(),
+                       lb.dreg.as<float4>(), lb.dlptr.as<int>(), lb.dlnb.as<int>(),
+                       p->attract_weight, p->repel_weight, p->damping, p->dt, iterations,
+                       init ? 1 : 0, lb.dx.as<float>(), lb.dy.as<float>(), lb.dvx.as<float>(),
+                       lb.dvy.as<float>(), lb.dfx.as<float>(), lb.dfy.as<float>());
   else
-    hipLaunchKernelGGL(place_leaves_kernel<false>, grid, block, 0, st, dbins.as<int4>(),
-                       dbase.as<int>(), dn.as<int>(), dreg.as<float4>(), dlptr.as<int>(),
-                       dlnb.as<int>(), p->attract_weight, p->repel_weight, p->damping, p->dt,
-                       iterations, init ? 1 : 0, dx.as<float>(), dy.as<float>(), dvx.as<float>(),
-                       dvy.as<float>(), dfx.as<float>(), dfy.as<float>());
+    hipLaunchKernelGGL(place_leaves_kernel<false>, grid, block, 0, st, lb.dwg.as<int>(),
+                       lb.dsegs.as<int4>(), lb.dbase.as<int>(), lb.dn.as<int>(),
+                       lb.dreg.as<float4>(), lb.dlptr.as<int>(), lb.dlnb.as<int>(),
+                       p->attract_weight, p->repel_weight, p->damping, p->dt, iterations,
+                       init ? 1 : 0, lb.dx.as<float>(), lb.dy.as<float>(), lb.dvx.as<float>(),
+                       lb.dvy.as<float>(), lb.dfx.as<float>(), lb.dfy.as<float>());
   FX_HIP(hipGetLastError());
   FX_HIP(hipEventRecord(ctx->ev1, st));
-  FX_HIP(hipMemcpyAsync(hx.data(), dx.p, 4 * P, hipMemcpyDeviceToHost, st));
-  FX_HIP(hipMemcpyAsync(hy.data(), dy.p, 4 * P, hipMemcpyDeviceToHost, st));
-  FX_HIP(hipMemcpyAsync(hvx.data(), dvx.p, 4 * P, hipMemcpyDeviceToHost, st));
-  FX_HIP(hipMemcpyAsync(hvy.data(), dvy.p, 4 * P, hipMemcpyDeviceToHost, st));
-  std::vector<float> hfx, hfy;
+  return FX_OK;
+}
+
+int elapsed_ms(fx_ctx* ctx, double* ms) {
+  FX_HIP(hipStreamSynchronize(ctx->stream));
+  float f = 0.f;
+  FX_HIP(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+  *ms = f;
+  return FX_OK;
+}
+
+int check_args(fx_ctx* ctx, const fx_leaf_batch* b, const fx_params* p, int32_t mode) {
+  FX_CHECK(ctx && b && p, FX_E_INVALID, "fx_place_leaves: NULL argument");
+  FX_CHECK(mode == FX_LEAF_EXACT || mode == FX_LEAF_FAST, FX_E_INVALID, "bad mode %d", mode);
+  FX_CHECK(b->num_leaves >= 0 && b->num_gates >= 0, FX_E_INVALID, "negative sizes");
+  return FX_OK;
+}
+
+}  // namespace
+
+extern "C" int fx_place_leaves_state(fx_ctx* ctx, const fx_leaf_batch* b, const fx_params* p,
+                                     int32_t mode, int32_t init, int32_t iterations, float* x,
+                                     float* y, float* vx, float* vy, float* fx, float* fy) {
+  fx::clear_error();
+  int rc = check_args(ctx, b, p, mode);
+  if (rc) return rc;
+  FX_CHECK(x && y, FX_E_INVALID, "fx_place_leaves: NULL positions");
+  FX_CHECK(iterations >= 0, FX_E_INVALID, "negative iteration count");
+  if (b->num_leaves == 0) return FX_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  FX_HIP(hipSetDevice(ctx->device));
+  LeafBatch lb;
+  if ((rc = prepare(ctx, b, init, x, y, vx, vy, fx && fy, lb))) return rc;
+  int ntiles = 0;
+  if ((rc = launch(ctx, lb, ctx->leaf_tile[mode], p, mode, init, iterations, &ntiles))) return rc;
+  if (ntiles == 0) return FX_OK;
+  const int P = lb.P;
+  std::vector<float> hx(P), hy(P), hvx(P), hvy(P), hfx, hfy;
+  hipStream_t st = ctx->stream;
+  FX_HIP(hipMemcpyAsync(hx.data(), lb.dx.p, 4 * P, hipMemcpyDeviceToHost, st));
+  FX_HIP(hipMemcpyAsync(hy.data(), lb.dy.p, 4 * P, hipMemcpyDeviceToHost, st));
+  FX_HIP(hipMemcpyAsync(hvx.data(), lb.dvx.p, 4 * P, hipMemcpyDeviceToHost, st));
+  FX_HIP(hipMemcpyAsync(hvy.data(), lb.dvy.p, 4 * P, hipMemcpyDeviceToHost, st));
   if (fx && fy) {
     hfx.resize(P);
     hfy.resize(P);
-    FX_HIP(hipMemcpyAsync(hfx.data(), dfx.p, 4 * P, hipMemcpyDeviceToHost, st));
-    FX_HIP(hipMemcpyAsync(hfy.data(), dfy.p, 4 * P, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipMemcpyAsync(hfx.data(), lb.dfx.p, 4 * P, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipMemcpyAsync(hfy.data(), lb.dfy.p, 4 * P, hipMemcpyDeviceToHost, st));
   }
-  FX_HIP(hipStreamSynchronize(st));
-  float ms = 0.f;
-  FX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  double ms = 0.0;
+  if ((rc = elapsed_ms(ctx, &ms))) return rc;
   ctx->leaves_ms = ms;
   for (int i = 0; i < P; ++i) {
     const int g = b->leaf_gates[i];
@@ -343,3 +428,62 @@ extern "C" int fx_place_leaves(fx_ctx* ctx, const fx_leaf_batch* batch, const fx
 }
 
 extern "C" double fx_leaves_last_kernel_ms(const fx_ctx* ctx) { return ctx ? ctx->leaves_ms : 0.0; }
+
+extern "C" int fx_leaves_set_tile(fx_ctx* ctx, int32_t mode, const fx_leaf_tile* tile) {
+  fx::clear_error();
+  FX_CHECK(ctx && tile, FX_E_INVALID, "fx_leaves_set_tile: NULL argument");
+  FX_CHECK(mode == FX_LEAF_EXACT || mode == FX_LEAF_FAST, FX_E_INVALID, "bad mode %d", mode);
+  FX_CHECK(tile_ok(*tile), FX_E_INVALID, "bad tile shape (%d gates/lane, %d min lanes, block %d)",
+           tile->gates_per_lane, tile->min_lanes, tile->block);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->leaf_tile[mode] = *tile;
+  return FX_OK;
+}
+
+extern "C" int fx_leaves_get_tile(fx_ctx* ctx, int32_t mode, fx_leaf_tile* tile) {
+  fx::clear_error();
+  FX_CHECK(ctx && tile, FX_E_INVALID, "fx_leaves_get_tile: NULL argument");
+  FX_CHECK(mode == FX_LEAF_EXACT || mode == FX_LEAF_FAST, FX_E_INVALID, "bad mode %d", mode);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  *tile = ctx->leaf_tile[mode];
+  return FX_OK;
+}
+
+extern "C" int fx_leaves_autotune(fx_ctx* ctx, const fx_leaf_batch* b, const fx_params* p,
+                                  int32_t mode, int32_t iterations, fx_leaf_tile* best,
+                                  double* best_ms) {
+  fx::clear_error();
+  int rc = check_args(ctx, b, p, mode);
+  if (rc) return rc;
+  FX_CHECK(iterations >= 1, FX_E_INVALID, "autotune needs at least one iteration");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  FX_HIP(hipSetDevice(ctx->device));
+  LeafBatch lb;
+  if ((rc = prepare(ctx, b, 1, nullptr, nullptr, nullptr, nullptr, false, lb))) return rc;
+  fx_leaf_tile win = ctx->leaf_tile[mode];
+  double win_ms = -1.0;
+  for (int block : {64, 128, 256})
+    for (int gpl : {1, 2, 4})
+      for (int ml : {8, 16, 32, 64}) {
+        const fx_leaf_tile t{gpl, ml, block};
+        std::vector<int> wg_ptr;
+        std::vector<int4> segs;
+        if (!build_tiles(lb.leaf_n, t, wg_ptr, segs)) continue;  // shape cannot hold a leaf
+        double ms = 0.0;
+        int ntiles = 0;
+        // Two runs (from the initial positions each time); the second is timed.
+        for (int rep = 0; rep < 2; ++rep) {
+          if ((rc = launch(ctx, lb, t, p, mode, 1, iterations, &ntiles)) ||
+              (rc = elapsed_ms(ctx, &ms)))
+            return rc;
+        }
+        if (win_ms < 0.0 || ms < win_ms) {
+          win_ms = ms;
+          win = t;
+        }
+      }
+  ctx->leaf_tile[mode] = win;
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return FX_OK;
+}

@@ -54,6 +54,10 @@ class FxParams(C.Structure):
     ]
 
 
+class FxLeafTile(C.Structure):
+    _fields_ = [("gates_per_lane", C.c_int32), ("min_lanes", C.c_int32), ("block", C.c_int32)]
+
+
 class FxNetlist(C.Structure):
     _fields_ = [
         ("num_gates", C.c_int32),
@@ -103,6 +107,10 @@ _SIGS = {
     "fx_place_leaves_state": (C.c_int, [_P, C.POINTER(FxLeafBatch), C.POINTER(FxParams), _I32, _I32,
                                         _I32, _P, _P, _P, _P, _P, _P]),
     "fx_leaves_last_kernel_ms": (_D, [_P]),
+    "fx_leaves_set_tile": (C.c_int, [_P, _I32, C.POINTER(FxLeafTile)]),
+    "fx_leaves_get_tile": (C.c_int, [_P, _I32, C.POINTER(FxLeafTile)]),
+    "fx_leaves_autotune": (C.c_int, [_P, C.POINTER(FxLeafBatch), C.POINTER(FxParams), _I32, _I32,
+                                     C.POINTER(FxLeafTile), C.POINTER(_D)]),
     "fx_job_create": (C.c_int, [_P, C.POINTER(FxNetlist), C.POINTER(FxParams), C.POINTER(_P)]),
     "fx_job_destroy": (None, [_P]),
     "fx_job_set_positions": (C.c_int, [_P, _P, _P, _I64]),

@@ -25,13 +25,13 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._lib import (FX_FLAG_FAST_MATH, FX_FLAG_NO_DENSITY, FX_FLAG_NO_REPULSION, FX_FLAG_NO_SPRINGS, FX_LEAF_EXACT,
-                   FX_LEAF_FAST, FluxionError, FxLeafBatch, FxNetlist, FxParams, FxServerConfig,
+                   FX_LEAF_FAST, FluxionError, FxLeafBatch, FxLeafTile, FxNetlist, FxParams, FxServerConfig,
                    check, default_params, f32p, i32p, lib, ptr)
 
 __all__ = [
     "PlacerConfig", "GpuContext", "Circuit", "Placement", "FlatJob", "PlacementServer",
-    "place_leaves", "dispatch_forces", "dispatch_congestion", "compute_fiedler_vector",
-    "split_by_fiedler", "sort_f32", "sort_u64", "FluxionError",
+    "place_leaves", "leaves_tile", "set_leaves_tile", "autotune_leaves", "dispatch_forces",
+    "dispatch_congestion", "compute_fiedler_vector", "split_by_fiedler", "sort_f32", "sort_u64", "FluxionError",
 ]
 
 
@@ -203,6 +203,32 @@ def place_leaves(ctx: GpuContext, leaf_ptr, leaf_gates, region, circuit: Circuit
                               FX_LEAF_FAST if fast else FX_LEAF_EXACT,
                               ptr(placement.gate_x), ptr(placement.gate_y)))
     return float(lib.fx_leaves_last_kernel_ms(ctx.handle))
+
+
+def leaves_tile(ctx: GpuContext, fast: bool = False) -> tuple:
+    """(gates_per_lane, min_lanes, block) used by the leaf launch for the mode."""
+    t = FxLeafTile()
+    check(lib.fx_leaves_get_tile(ctx.handle, FX_LEAF_FAST if fast else FX_LEAF_EXACT, C.byref(t)))
+    return (t.gates_per_lane, t.min_lanes, t.block)
+
+
+def set_leaves_tile(ctx: GpuContext, tile, fast: bool = False) -> None:
+    t = FxLeafTile(*tile)
+    check(lib.fx_leaves_set_tile(ctx.handle, FX_LEAF_FAST if fast else FX_LEAF_EXACT, C.byref(t)))
+
+
+def autotune_leaves(ctx: GpuContext, leaf_ptr, leaf_gates, region, circuit: Circuit,
+                    config: PlacerConfig, iterations: int = 10, fast: bool = False):
+    """Times every tile shape on this batch and keeps the fastest for the mode;
+    returns ((gates_per_lane, min_lanes, block), ms)."""
+    b, keep = _leaf_batch(leaf_ptr, leaf_gates, region, circuit)
+    p = config.to_params()
+    t = FxLeafTile()
+    ms = C.c_double()
+    check(lib.fx_leaves_autotune(ctx.handle, C.byref(b), C.byref(p),
+                                 FX_LEAF_FAST if fast else FX_LEAF_EXACT, int(iterations), C.byref(t),
+                                 C.byref(ms)))
+    return (t.gates_per_lane, t.min_lanes, t.block), ms.value
 
 
 def place_leaves_state(ctx: GpuContext, leaf_ptr, leaf_gates, region, circuit: Circuit,

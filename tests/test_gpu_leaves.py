@@ -113,3 +113,79 @@ def test_invalid_inputs_rejected(ctx):
     dup[1] = dup[0]
     with pytest.raises(fxn.FluxionError):
         fxn.place_leaves(ctx, lp, dup, reg, circ, pl, fxn.PlacerConfig())
+
+
+DEFAULT_TILE = (1, 64, 256)
+
+
+def _edge_batch():
+    sizes = [1, 2, 7, 9, 15, 16, 17, 31, 33, 63, 64, 65, 100, 129, 191, 193, 255, 256, 3, 40]
+    nets, lp, start = [], [0], 0
+    rng = np.random.default_rng(11)
+    for n in sizes:
+        for _ in range(n):
+            if n >= 2:
+                nets.append(list(start + rng.choice(n, size=min(n, 3), replace=False)))
+        start += n
+        lp.append(start)
+    circ = fxn.Circuit.from_nets(start, nets)
+    reg = np.array([[50 + 100 * i, 50, 100, 100] for i in range(len(sizes))], np.float32)
+    return circ, np.array(lp, np.int32), np.arange(start, dtype=np.int32), reg
+
+
+@pytest.mark.parametrize("tile", [(1, 8, 256), (2, 16, 128), (4, 32, 64), (1, 64, 64),
+                                  (2, 8, 256), (4, 8, 128), (4, 64, 256)])
+def test_exact_bit_identical_for_every_tile_shape(ctx, tile):
+    # Segments of 8..256 lanes, several leaves per wave, 1-4 gates per lane:
+    # the exact per-gate summation order does not depend on the tile shape.
+    print(f"case: leaf tile {tile}", flush=True)
+    try:
+        fxn.set_leaves_tile(ctx, tile)
+        assert fxn.leaves_tile(ctx) == tile
+        for circ, lp, gates, reg in (_edge_batch(), fxn.netgen.leaves(300, 1300)):
+            pl = fxn.Placement.new(circ.num_gates)
+            fxn.place_leaves(ctx, lp, gates, reg, circ, pl, fxn.PlacerConfig(force_iterations=12))
+            aptr, adj = circ.build_adjacency()
+            ox, oy, *_ = orc.place_leaves(lp, gates, reg, circ.num_gates, aptr, adj, iters=12)
+            assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
+    finally:
+        fxn.set_leaves_tile(ctx, DEFAULT_TILE)
+
+
+def test_autotune_picks_a_shape_and_results_stay_exact(ctx):
+    print("case: leaf autotune", flush=True)
+    circ, lp, gates, reg = fxn.netgen.leaves(512, 1400)
+    try:
+        for fast in (False, True):
+            tile, ms = fxn.autotune_leaves(ctx, lp, gates, reg, circ,
+                                           fxn.PlacerConfig(force_iterations=5), iterations=5,
+                                           fast=fast)
+            print(f"autotune fast={fast}: {tile} {ms:.3f} ms", flush=True)
+            assert ms > 0 and fxn.leaves_tile(ctx, fast) == tile
+        pl = fxn.Placement.new(circ.num_gates)
+        fxn.place_leaves(ctx, lp, gates, reg, circ, pl, fxn.PlacerConfig(force_iterations=20))
+        aptr, adj = circ.build_adjacency()
+        ox, oy, *_ = orc.place_leaves(lp, gates, reg, circ.num_gates, aptr, adj, iters=20)
+        assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
+    finally:
+        fxn.set_leaves_tile(ctx, DEFAULT_TILE)
+        fxn.set_leaves_tile(ctx, DEFAULT_TILE, fast=True)
+
+
+def test_tile_shape_rejections(ctx):
+    print("case: leaf tile rejections", flush=True)
+    for bad in [(3, 8, 256), (1, 4, 256), (1, 8, 512), (0, 0, 0)]:
+        with pytest.raises(fxn.FluxionError):
+            fxn.set_leaves_tile(ctx, bad)
+    assert fxn.leaves_tile(ctx) == DEFAULT_TILE
+    # a 600-gate leaf needs more than 4 gates per lane in a 64-thread workgroup
+    nets = [[i, i + 1] for i in range(599)]
+    circ = fxn.Circuit.from_nets(600, nets)
+    reg = np.array([[50, 50, 100, 100]], np.float32)
+    try:
+        fxn.set_leaves_tile(ctx, (1, 64, 64))
+        with pytest.raises(fxn.FluxionError):
+            fxn.place_leaves(ctx, [0, 600], np.arange(600, dtype=np.int32), reg, circ,
+                             fxn.Placement.new(600), fxn.PlacerConfig(force_iterations=1))
+    finally:
+        fxn.set_leaves_tile(ctx, DEFAULT_TILE)
