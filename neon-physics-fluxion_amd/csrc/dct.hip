@@ -33,7 +33,9 @@
 // the density fluctuations rather than to the background.
 // A workgroup of four waves owns a 32x32 output tile, each wave a quarter of
 // the K range (summed through LDS in fixed order); operands for the next K
-// step are loaded while the current step's MFMAs issue. Bin counts are
+// step are loaded while the current step's MFMAs issue. The two independent
+// products of each inverse stage (U and V, then Ex and Ey) share one launch,
+// so an iteration issues four GEMM launches of 256-512 workgroups. Bin counts are
 // deposited with int32 atomics (exact, order-free).
 
 #include <cmath>
@@ -134,12 +136,13 @@ __device__ __forceinline__ void load_operand(const GemmArgs& g, int idx_row, int
 // order (deterministic).
 constexpr int kGemmWaves = 4;
 
+// One 32x32 output tile of one product (tile index `tile`).
 template <int AK, int BK, int EPI, int SPLIT>
-__global__ __launch_bounds__(64 * kGemmWaves) void dct_gemm(GemmArgs g) {
-  __shared__ float part[kGemmWaves - 1][16][64];
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile,
+                                          float (*part)[16][64]) {
   const int G = g.G;
   const int tiles_n = G / 32;
-  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  const int tm = tile / tiles_n, tn = tile % tiles_n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 31;         // row of the A fragment / column of the B fragment
   const int kq = (lane >> 5) * 4;  // k offset inside the 8-wide step
@@ -190,6 +193,22 @@ __global__ __launch_bounds__(64 * kGemmWaves) void dct_gemm(GemmArgs g) {
     if (EPI == kEigen) v = (row == 0 && col == 0) ? 0.0f : -v / g.eig[o];
     g.out[o] = v;
   }
+}
+
+template <int AK, int BK, int EPI, int SPLIT>
+__global__ __launch_bounds__(64 * kGemmWaves) void dct_gemm(GemmArgs g) {
+  __shared__ float part[kGemmWaves - 1][16][64];
+  gemm_tile<AK, BK, EPI, SPLIT>(g, blockIdx.x, part);
+}
+
+// Two independent products in one launch: the first G^2/1024 workgroups
+// compute product 0, the rest product 1.
+template <int AK0, int BK0, int AK1, int BK1>
+__global__ __launch_bounds__(64 * kGemmWaves) void dct_gemm_pair(GemmArgs g0, GemmArgs g1) {
+  __shared__ float part[kGemmWaves - 1][16][64];
+  const int tiles = (g0.G / 32) * (g0.G / 32);
+  if ((int)blockIdx.x < tiles) gemm_tile<AK0, BK0, kStore, 3>(g0, blockIdx.x, part);
+  else gemm_tile<AK1, BK1, kStore, 3>(g1, blockIdx.x - tiles, part);
 }
 #undef MFMA
 
@@ -262,6 +281,12 @@ template <int AK, int BK, int EPI, int SPLIT>
 void launch(const GemmArgs& g, hipStream_t st) {
   const dim3 grid((unsigned)((g.G / 32) * (g.G / 32))), block(64 * kGemmWaves);
   hipLaunchKernelGGL((dct_gemm<AK, BK, EPI, SPLIT>), grid, block, 0, st, g);
+}
+
+template <int AK0, int BK0, int AK1, int BK1>
+void launch_pair(const GemmArgs& g0, const GemmArgs& g1, hipStream_t st) {
+  const dim3 grid((unsigned)(2 * (g0.G / 32) * (g0.G / 32))), block(64 * kGemmWaves);
+  hipLaunchKernelGGL((dct_gemm_pair<AK0, BK0, AK1, BK1>), grid, block, 0, st, g0, g1);
 }
 
 }  // namespace
@@ -376,30 +401,28 @@ int dct_field(Job& j, bool counts_cleared) {
   g2.out = d.t1.as<float>();
   if (six) launch<kData, kConst, kEigen, 3>(g2, st);
   else launch<kData, kConst, kEigen, 2>(g2, st);
-  // U = C^T phi_hat
+  // U = C^T phi_hat and V = phi_hat C (right operand C stored transposed:
+  // C^T), one launch
   GemmArgs g3 = a;
   for (int k = 0; k < 3; ++k) g3.a_p[k] = Ct[k];
   g3.b_f = d.t1.as<float>();
   g3.out = d.t0.as<float>();
-  launch<kConst, kData, kStore, 3>(g3, st);
-  // Ex = U (C Dx) = centred x-differences of phi = U C
-  GemmArgs g4 = a;
-  g4.a_f = d.t0.as<float>();
-  for (int k = 0; k < 3; ++k) g4.bt_p[k] = Dxt[k];
-  g4.out = d.ex.as<float>();
-  launch<kData, kConst, kStore, 3>(g4, st);
-  // V = phi_hat C   (right operand C stored transposed: C^T)
   GemmArgs g5 = a;
   g5.a_f = d.t1.as<float>();
   for (int k = 0; k < 3; ++k) g5.bt_p[k] = Ct[k];
   g5.out = d.t2.as<float>();
-  launch<kData, kConst, kStore, 3>(g5, st);
-  // Ey = (Dy C^T) V = centred y-differences of phi = C^T V
+  launch_pair<kConst, kData, kData, kConst>(g3, g5, st);
+  // Ex = U (C Dx) (centred x-differences of phi = U C) and
+  // Ey = (Dy C^T) V (centred y-differences of phi = C^T V), one launch
+  GemmArgs g4 = a;
+  g4.a_f = d.t0.as<float>();
+  for (int k = 0; k < 3; ++k) g4.bt_p[k] = Dxt[k];
+  g4.out = d.ex.as<float>();
   GemmArgs g6 = a;
   for (int k = 0; k < 3; ++k) g6.a_p[k] = Dy[k];
   g6.b_f = d.t2.as<float>();
   g6.out = d.ey.as<float>();
-  launch<kConst, kData, kStore, 3>(g6, st);
+  launch_pair<kData, kConst, kConst, kData>(g4, g6, st);
   FX_HIP(hipGetLastError());
   return FX_OK;
 }
