@@ -31,9 +31,10 @@
 // DC coefficient, and the DC coefficient is discarded (phi_hat[0][0] = 0), so
 // phi is unchanged; removing the mean keeps the split precision relative to
 // the density fluctuations rather than to the background.
-// A workgroup of four waves owns a 32x32 output tile, each wave a quarter of
+// A workgroup owns a 32x32 output tile, each of its waves an equal share of
 // the K range (summed through LDS in fixed order); operands for the next K
-// step are loaded while the current step's MFMAs issue. The two independent
+// step are loaded while the current step's MFMAs issue (eight waves for
+// G >= 256). The two independent
 // products of each inverse stage (U and V, then Ex and Ey) share one launch,
 // so an iteration issues four GEMM launches of 256-512 workgroups. Bin counts are
 // deposited with int32 atomics (exact, order-free).
@@ -131,13 +132,13 @@ __device__ __forceinline__ void load_operand(const GemmArgs& g, int idx_row, int
 
 #define MFMA(a, b) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0)
 
-// One workgroup (4 waves) per 32x32 output tile; wave w reduces the K range
-// [w*G/4, (w+1)*G/4) and the partial tiles are summed through LDS in a fixed
-// order (deterministic).
-constexpr int kGemmWaves = 4;
+// One workgroup (NW waves: 8 for G >= 256, else 4) per 32x32 output tile;
+// wave w reduces the K range [w*G/NW, (w+1)*G/NW) and the partial tiles are
+// summed through LDS in a fixed order (deterministic).
+inline int gemm_waves(int G) { return G >= 256 ? 8 : 4; }
 
 // One 32x32 output tile of one product (tile index `tile`).
-template <int AK, int BK, int EPI, int SPLIT>
+template <int AK, int BK, int EPI, int SPLIT, int NW>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile,
                                           float (*part)[16][64]) {
   const int G = g.G;
@@ -148,7 +149,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile,
   const int kq = (lane >> 5) * 4;  // k offset inside the 8-wide step
   const int arow = tm * 32 + r;
   const int bcol = tn * 32 + r;
-  const int kspan = G / kGemmWaves;
+  const int kspan = G / NW;
   const int kbeg = wave * kspan, kend = kbeg + kspan;
   float16_t acc = {};
   short4_t a0, a1, a2, b0, b1, b2;
@@ -186,7 +187,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile,
   for (int e = 0; e < 16; ++e) {
     float v = acc[e];
 #pragma unroll
-    for (int w = 0; w < kGemmWaves - 1; ++w) v += part[w][e][lane];
+    for (int w = 0; w < NW - 1; ++w) v += part[w][e][lane];
     const int row = tm * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
     const int col = tn * 32 + (lane & 31);
     const size_t o = (size_t)row * G + col;
@@ -195,20 +196,20 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile,
   }
 }
 
-template <int AK, int BK, int EPI, int SPLIT>
-__global__ __launch_bounds__(64 * kGemmWaves) void dct_gemm(GemmArgs g) {
-  __shared__ float part[kGemmWaves - 1][16][64];
-  gemm_tile<AK, BK, EPI, SPLIT>(g, blockIdx.x, part);
+template <int AK, int BK, int EPI, int SPLIT, int NW>
+__global__ __launch_bounds__(64 * NW) void dct_gemm(GemmArgs g) {
+  __shared__ float part[NW - 1][16][64];
+  gemm_tile<AK, BK, EPI, SPLIT, NW>(g, blockIdx.x, part);
 }
 
 // Two independent products in one launch: the first G^2/1024 workgroups
 // compute product 0, the rest product 1.
-template <int AK0, int BK0, int AK1, int BK1>
-__global__ __launch_bounds__(64 * kGemmWaves) void dct_gemm_pair(GemmArgs g0, GemmArgs g1) {
-  __shared__ float part[kGemmWaves - 1][16][64];
+template <int AK0, int BK0, int AK1, int BK1, int NW>
+__global__ __launch_bounds__(64 * NW) void dct_gemm_pair(GemmArgs g0, GemmArgs g1) {
+  __shared__ float part[NW - 1][16][64];
   const int tiles = (g0.G / 32) * (g0.G / 32);
-  if ((int)blockIdx.x < tiles) gemm_tile<AK0, BK0, kStore, 3>(g0, blockIdx.x, part);
-  else gemm_tile<AK1, BK1, kStore, 3>(g1, blockIdx.x - tiles, part);
+  if ((int)blockIdx.x < tiles) gemm_tile<AK0, BK0, kStore, 3, NW>(g0, blockIdx.x, part);
+  else gemm_tile<AK1, BK1, kStore, 3, NW>(g1, blockIdx.x - tiles, part);
 }
 #undef MFMA
 
@@ -279,14 +280,20 @@ int upload3(hipStream_t st, const std::vector<double>& m, DevBuf* dst[3]) {
 
 template <int AK, int BK, int EPI, int SPLIT>
 void launch(const GemmArgs& g, hipStream_t st) {
-  const dim3 grid((unsigned)((g.G / 32) * (g.G / 32))), block(64 * kGemmWaves);
-  hipLaunchKernelGGL((dct_gemm<AK, BK, EPI, SPLIT>), grid, block, 0, st, g);
+  const dim3 grid((unsigned)((g.G / 32) * (g.G / 32)));
+  if (gemm_waves(g.G) == 8)
+    hipLaunchKernelGGL((dct_gemm<AK, BK, EPI, SPLIT, 8>), grid, dim3(512), 0, st, g);
+  else
+    hipLaunchKernelGGL((dct_gemm<AK, BK, EPI, SPLIT, 4>), grid, dim3(256), 0, st, g);
 }
 
 template <int AK0, int BK0, int AK1, int BK1>
 void launch_pair(const GemmArgs& g0, const GemmArgs& g1, hipStream_t st) {
-  const dim3 grid((unsigned)(2 * (g0.G / 32) * (g0.G / 32))), block(64 * kGemmWaves);
-  hipLaunchKernelGGL((dct_gemm_pair<AK0, BK0, AK1, BK1>), grid, block, 0, st, g0, g1);
+  const dim3 grid((unsigned)(2 * (g0.G / 32) * (g0.G / 32)));
+  if (gemm_waves(g0.G) == 8)
+    hipLaunchKernelGGL((dct_gemm_pair<AK0, BK0, AK1, BK1, 8>), grid, dim3(512), 0, st, g0, g1);
+  else
+    hipLaunchKernelGGL((dct_gemm_pair<AK0, BK0, AK1, BK1, 4>), grid, dim3(256), 0, st, g0, g1);
 }
 
 }  // namespace
