@@ -410,6 +410,14 @@ def main():
             out["graph_replay"] = {"iters_per_s": K / (time.perf_counter() - t0),
                                    "final_hpwl": hg, "same_hpwl_as_eager": hg == hpwl}
             job.use_graph(False)
+        # Congestion-aware refinement (refiner.rs:34-95, RefineConfig defaults) of the
+        # final placement: the ROUTE step that follows global placement.
+        job.sync()
+        t0 = time.perf_counter()
+        ref = job.refine(fxn.RefineConfig(grid_cols=C, grid_rows=C))
+        ref["seconds"] = time.perf_counter() - t0
+        ref["ms_per_iteration"] = ref["seconds"] * 1e3 / max(1, ref["iterations"])
+        out["refine"] = ref
         out["leaves_4096"] = leaves_workload(fxn, ctx)
         out["flat_2m"] = flat2m_workload(fxn, ctx)
     if args.serve_requests > 0:

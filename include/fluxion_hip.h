@@ -198,6 +198,39 @@ int fx_density_overflow(fx_job* job, int32_t bins, double* max_density, double* 
 int fx_dispatch_congestion(fx_ctx* ctx, const fx_netlist* net, const float* x,
                            const float* y, int32_t cols, int32_t rows, int32_t* counts);
 
+/* ---- congestion-aware refinement (refine_placement, refiner.rs:34-95) ----
+ * Runs the reference's refinement loop on the job's device-resident
+ * positions: per iteration the routing grid and its FP32 total overflow
+ * (row-major sum, grid.rs:85-91), the convergence (<= threshold) and stall
+ * (10 iterations changing by < 1%) tests, then every gate in a hot cell moves
+ * toward the least congested cell within 3 cells, by at most move_distance
+ * per axis, clamped to the die (move_gate_from_hot_cell, 98-125). Positions
+ * are identical to the reference's sequential loop. */
+typedef struct fx_refine_config {  /* RefineConfig (refiner.rs:13-19) */
+  int32_t max_iterations;
+  float move_distance;
+  float overflow_threshold;
+  int32_t grid_cols;
+  int32_t grid_rows;
+} fx_refine_config;
+
+#define FX_REFINE_MAX_ITERATIONS 0 /* ran max_iterations */
+#define FX_REFINE_CONVERGED 1      /* total overflow <= threshold (refiner.rs:54-57) */
+#define FX_REFINE_STALLED 2        /* 10 iterations with < 1% change (60-66) */
+
+typedef struct fx_refine_result {
+  int32_t iterations;   /* loop index at exit (moves applied in that many) */
+  int32_t stop;         /* FX_REFINE_* */
+  float total_overflow; /* final grid, FP32 as grid.rs:85-91 */
+  float max_overflow;   /* final grid (Placement::max_congestion, refiner.rs:86) */
+  double hpwl;          /* FP64 HPWL after refinement */
+  int64_t gate_moves;   /* moves applied over all iterations */
+} fx_refine_result;
+
+/* RefineConfig::default (refiner.rs:21-31): 100, 5.0, 0.0, 256, 256. */
+void fx_refine_defaults(fx_refine_config* cfg);
+int fx_job_refine(fx_job* job, const fx_refine_config* cfg, fx_refine_result* out);
+
 /* ---- power iteration (compute_fiedler_vector, partitioner.rs:106-185) ----
  * Laplacian of the subgraph induced by gates[0..n) over the global adjacency
  * CSR (partitioner.rs:116-137), FP32 SpMV in the reference's summation order,

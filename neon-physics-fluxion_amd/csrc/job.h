@@ -112,6 +112,9 @@ int combine(Job& j, int components, bool integrate);  // forces -> (fxo,fyo) or 
 int bh_alloc(Job& j);
 int dct_alloc(Job& j);
 int bh_grow(Job& j, int64_t needed);
+// Routing-grid counts (h = v, congestion.rs:11-62) of the job's positions;
+// *out_counts points to rows*cols int32 in job-owned device memory.
+int congestion_counts(Job& j, int cols, int rows, int32_t** out_counts);
 // Clears the overflow flag (set by any iteration of a run that exceeded the
 // node capacity; stays set until the next run starts).
 int bh_clear_overflow(Job& j);

@@ -2,7 +2,7 @@
 reference's placer / GPU-operator API. Compute runs in libfluxion_hip.so."""
 from ._lib import FluxionError, LIB_PATH, lib  # noqa: F401  (raises if the library is missing)
 from .placer import (Circuit, FlatJob, GpuContext, Placement, PlacementServer,  # noqa: F401
-                     PlacerConfig, autotune_leaves, compute_fiedler_vector, dispatch_congestion,
+                     PlacerConfig, RefineConfig, autotune_leaves, compute_fiedler_vector, dispatch_congestion,
                      dispatch_forces, fiedler_init, leaves_tile, place_leaves, place_leaves_state,
-                     set_leaves_tile, sort_f32, sort_u64, split_by_fiedler)
+                     refine_placement, set_leaves_tile, sort_f32, sort_u64, split_by_fiedler)
 from . import netgen  # noqa: F401

@@ -58,6 +58,16 @@ class FxLeafTile(C.Structure):
     _fields_ = [("gates_per_lane", C.c_int32), ("min_lanes", C.c_int32), ("block", C.c_int32)]
 
 
+class FxRefineConfig(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("move_distance", C.c_float),
+                ("overflow_threshold", C.c_float), ("grid_cols", C.c_int32), ("grid_rows", C.c_int32)]
+
+
+class FxRefineResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("stop", C.c_int32), ("total_overflow", C.c_float),
+                ("max_overflow", C.c_float), ("hpwl", C.c_double), ("gate_moves", C.c_int64)]
+
+
 class FxNetlist(C.Structure):
     _fields_ = [
         ("num_gates", C.c_int32),
@@ -131,6 +141,8 @@ _SIGS = {
     "fx_overflow": (C.c_int, [_P, _I32, _I32, C.POINTER(_D), C.POINTER(_D)]),
     "fx_density_overflow": (C.c_int, [_P, _I32, C.POINTER(_D), C.POINTER(_D)]),
     "fx_dispatch_congestion": (C.c_int, [_P, C.POINTER(FxNetlist), _P, _P, _I32, _I32, _P]),
+    "fx_refine_defaults": (None, [C.POINTER(FxRefineConfig)]),
+    "fx_job_refine": (C.c_int, [_P, C.POINTER(FxRefineConfig), C.POINTER(FxRefineResult)]),
     "fx_fiedler": (C.c_int, [_P, _I32, _P, _I32, _P, _P, _P, _I32, _P, C.POINTER(_D)]),
     "fx_fiedler_last_ms": (_D, [_P]),
     "fx_sort_u64": (C.c_int, [_P, _I64, _P, _P]),

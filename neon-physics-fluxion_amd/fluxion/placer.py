@@ -25,13 +25,15 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._lib import (FX_FLAG_FAST_MATH, FX_FLAG_NO_DENSITY, FX_FLAG_NO_REPULSION, FX_FLAG_NO_SPRINGS, FX_LEAF_EXACT,
-                   FX_LEAF_FAST, FluxionError, FxLeafBatch, FxLeafTile, FxNetlist, FxParams, FxServerConfig,
+                   FX_LEAF_FAST, FluxionError, FxLeafBatch, FxLeafTile, FxRefineConfig,
+                   FxRefineResult, FxNetlist, FxParams, FxServerConfig,
                    check, default_params, f32p, i32p, lib, ptr)
 
 __all__ = [
     "PlacerConfig", "GpuContext", "Circuit", "Placement", "FlatJob", "PlacementServer",
     "place_leaves", "leaves_tile", "set_leaves_tile", "autotune_leaves", "dispatch_forces",
-    "dispatch_congestion", "compute_fiedler_vector", "split_by_fiedler", "sort_f32", "sort_u64", "FluxionError",
+    "dispatch_congestion", "compute_fiedler_vector", "split_by_fiedler", "RefineConfig",
+    "refine_placement", "sort_f32", "sort_u64", "FluxionError",
 ]
 
 
@@ -68,6 +70,23 @@ class PlacerConfig:
             min_dist=self.min_dist, density_weight=self.density_weight,
             target_density=self.target_density, density_grid=int(self.density_grid),
             dct_passes=int(self.dct_passes), flags=flags)
+
+
+@dataclass
+class RefineConfig:
+    """RefineConfig of refiner.rs:13-31 (field names and defaults)."""
+    max_iterations: int = 100
+    move_distance: float = 5.0
+    overflow_threshold: float = 0.0
+    grid_cols: int = 256
+    grid_rows: int = 256
+
+    def to_c(self) -> FxRefineConfig:
+        return FxRefineConfig(self.max_iterations, self.move_distance, self.overflow_threshold,
+                              self.grid_cols, self.grid_rows)
+
+
+REFINE_STOP = {0: "max_iterations", 1: "converged", 2: "stalled"}
 
 
 class GpuContext:
@@ -352,6 +371,15 @@ class FlatJob:
         check(lib.fx_job_read_timing(self._h, ptr(ms), ptr(cnt)))
         return {ph: (float(ms[k]), int(cnt[k])) for k, ph in enumerate(self.PHASES) if cnt[k]}
 
+    def refine(self, config: "RefineConfig | None" = None) -> dict:
+        """refine_placement (refiner.rs:34-95) on the job's positions."""
+        c = (config or RefineConfig()).to_c()
+        r = FxRefineResult()
+        check(lib.fx_job_refine(self._h, C.byref(c), C.byref(r)))
+        return {"iterations": r.iterations, "stop": REFINE_STOP.get(r.stop, r.stop),
+                "total_overflow": r.total_overflow, "max_overflow": r.max_overflow,
+                "hpwl": r.hpwl, "gate_moves": r.gate_moves}
+
     def congestion(self, cols: int, rows: int) -> np.ndarray:
         out = np.zeros((rows, cols), np.int32)
         check(lib.fx_congestion(self._h, cols, rows, ptr(out)))
@@ -380,6 +408,20 @@ def dispatch_forces(ctx: GpuContext, circuit: Circuit, placement: Placement,
         job.set_positions(placement.gate_x, placement.gate_y)
         job.run(iterations)
         placement.gate_x[:], placement.gate_y[:] = job.positions()
+
+
+def refine_placement(ctx: GpuContext, placement: Placement, circuit: Circuit,
+                     config: "RefineConfig | None" = None) -> dict:
+    """refine_placement(&mut Placement, &Circuit, &RefineConfig) (refiner.rs:34-39):
+    updates placement in place and returns the run's summary."""
+    cfg = PlacerConfig(density_grid=0, repulsion=False, density=False)
+    with FlatJob(ctx, circuit, cfg) as job:
+        job.set_positions(placement.gate_x, placement.gate_y)
+        res = job.refine(config)
+        x, y = job.positions()
+    placement.gate_x[:] = x
+    placement.gate_y[:] = y
+    return res
 
 
 def dispatch_congestion(ctx: GpuContext, cols: int, rows: int, placement: Placement,

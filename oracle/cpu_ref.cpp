@@ -318,6 +318,101 @@ ORC_API void orc_overflow_f64(const int32_t* counts, int32_t cols, int32_t rows,
   *maxo = m;
 }
 
+// refine_placement (refiner.rs:34-95) with move_gate_from_hot_cell (98-125),
+// gates_in_hot_cells (congestion.rs:77-93) and RoutingGrid (grid.rs:23-135),
+// literally: f32 h and v grids, hot cells listed before the gate loop, moves
+// applied to gates in ascending order. x, y are updated in place.
+ORC_API void orc_refine(int32_t num_gates, int32_t num_nets, const int32_t* net_ptr,
+                        const int32_t* net_pins, float chip_w, float chip_h, float* x, float* y,
+                        int32_t max_iterations, float move_distance, float threshold,
+                        int32_t cols, int32_t rows, int32_t* iterations, int32_t* stop,
+                        float* total_overflow, float* max_overflow) {
+  const size_t cells = (size_t)cols * rows;
+  const float cw = chip_w / (float)cols, ch = chip_h / (float)rows;
+  const float cap = (cw + ch) * 0.5f;
+  std::vector<int32_t> cnt(cells);
+  auto overflow_at = [&](size_t idx) {
+    const float h = (float)cnt[idx], v = (float)cnt[idx];
+    return std::fmax((h + v) - cap, 0.0f);
+  };
+  auto totals = [&](float* t, float* m) {
+    float tt = 0.0f, mm = 0.0f;
+    for (int32_t r = 0; r < rows; ++r)
+      for (int32_t c = 0; c < cols; ++c) {
+        const float o = overflow_at((size_t)r * cols + c);
+        tt += o;
+        mm = std::fmax(mm, o);
+      }
+    *t = tt;
+    *m = mm;
+  };
+  auto cell_at = [&](float px, float py, int64_t& c, int64_t& r) {
+    c = (int64_t)std::min<uint64_t>(rust_as_usize(px / cw), (uint64_t)cols - 1);
+    r = (int64_t)std::min<uint64_t>(rust_as_usize(py / ch), (uint64_t)rows - 1);
+  };
+  auto clampf = [](float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  float prev = std::numeric_limits<float>::max();
+  int32_t stall = 0, iter = 0, why = 0;
+  float t = 0.0f, m = 0.0f;
+  std::vector<char> hot(cells);
+  for (iter = 0; iter < max_iterations; ++iter) {
+    orc_congestion(num_nets, net_ptr, net_pins, x, y, chip_w, chip_h, cols, rows, cnt.data());
+    totals(&t, &m);
+    if (t <= threshold) {
+      why = 1;
+      break;
+    }
+    if (std::fabs(prev - t) < 0.01f * std::fmax(prev, 1.0f)) {
+      if (++stall >= 10) {
+        why = 2;
+        break;
+      }
+    } else {
+      stall = 0;
+    }
+    prev = t;
+    for (size_t i = 0; i < cells; ++i) hot[i] = overflow_at(i) > threshold;
+    std::vector<int32_t> hot_gates;
+    for (int32_t g = 0; g < num_gates; ++g) {
+      int64_t c, r;
+      cell_at(x[g], y[g], c, r);
+      if (hot[(size_t)r * cols + c]) hot_gates.push_back(g);
+    }
+    for (int32_t g : hot_gates) {
+      const float gx = x[g], gy = y[g];
+      int64_t col, row;
+      cell_at(gx, gy, col, row);
+      int64_t bc = -1, br = -1;
+      float best = 0.0f;
+      for (int64_t dr = -3; dr <= 3; ++dr)
+        for (int64_t dc = -3; dc <= 3; ++dc) {
+          if (dr == 0 && dc == 0) continue;
+          const int64_t nc = col + dc, nr = row + dr;
+          if (nc < 0 || nc >= cols || nr < 0 || nr >= rows) continue;
+          const size_t idx = (size_t)nr * cols + nc;
+          const float tc = (float)cnt[idx] + (float)cnt[idx];
+          if (bc < 0 || tc < best) {  // min_by keeps the first minimum
+            best = tc;
+            bc = nc;
+            br = nr;
+          }
+        }
+      if (bc < 0) continue;
+      const float tx = ((float)bc + 0.5f) * cw, ty = ((float)br + 0.5f) * ch;
+      const float dx = clampf(tx - gx, -move_distance, move_distance);
+      const float dy = clampf(ty - gy, -move_distance, move_distance);
+      x[g] = clampf(gx + dx, 0.0f, chip_w);
+      y[g] = clampf(gy + dy, 0.0f, chip_h);
+    }
+  }
+  orc_congestion(num_nets, net_ptr, net_pins, x, y, chip_w, chip_h, cols, rows, cnt.data());
+  totals(&t, &m);
+  *iterations = iter;
+  *stop = why;
+  *total_overflow = t;
+  *max_overflow = m;
+}
+
 // Placement::compute_hpwl (placement.rs:48-79): f32 accumulation as the
 // reference, and an FP64 oracle (all arithmetic in double).
 ORC_API float orc_hpwl_f32(int32_t num_nets, const int32_t* net_ptr,

@@ -22,6 +22,8 @@ _SIGS = {
                                     _P, _P, _P, _P, _P, _P, _I32]),
     "orc_congestion": (_I32, [_I32, _P, _P, _P, _P, _F, _F, _I32, _I32, _P]),
     "orc_overflow_f32": (None, [_P, _I32, _I32, _F, _F, _P, _P]),
+    "orc_refine": (None, [_I32, _I32, _P, _P, _F, _F, _P, _P, _I32, _F, _F, _I32, _I32, _P, _P, _P,
+                          _P]),
     "orc_overflow_f64": (None, [_P, _I32, _I32, _F, _F, _P, _P]),
     "orc_hpwl_f32": (_F, [_I32, _P, _P, _P, _P]),
     "orc_hpwl_f64": (_D, [_I32, _P, _P, _P, _P]),
@@ -123,6 +125,19 @@ def overflow(counts, chip_w, chip_h, dtype=np.float64):
         t, m = C.c_double(), C.c_double()
         lib.orc_overflow_f64(p(c), cols, rows, chip_w, chip_h, C.byref(t), C.byref(m))
     return t.value, m.value
+
+
+def refine(net_ptr, net_pins, chip_w, chip_h, x, y, max_iterations=100, move_distance=5.0,
+           threshold=0.0, cols=256, rows=256):
+    """refine_placement (refiner.rs:34-95): returns (x, y, iterations, stop, total, max)."""
+    net_ptr, net_pins = i32(net_ptr), i32(net_pins)
+    x, y = f32(x).copy(), f32(y).copy()
+    it, st = C.c_int32(), C.c_int32()
+    t, m = C.c_float(), C.c_float()
+    lib.orc_refine(x.shape[0], net_ptr.shape[0] - 1, p(net_ptr), p(net_pins), chip_w, chip_h,
+                   p(x), p(y), max_iterations, move_distance, threshold, cols, rows,
+                   C.byref(it), C.byref(st), C.byref(t), C.byref(m))
+    return x, y, it.value, st.value, t.value, m.value
 
 
 def hpwl(net_ptr, net_pins, x, y, dtype=np.float64):

@@ -278,3 +278,84 @@ def test_density_linearity_and_dc_removal():
     np.testing.assert_allclose(pab, pa + pb, atol=1e-9 * np.abs(pab).max())
     _, _, _, p1, _, _ = orc.density(xa, ya, die, die, G, target=0.8, fields=True)
     np.testing.assert_allclose(pa, p1, atol=1e-9 * np.abs(pa).max())
+
+
+# ---- refine_placement (refiner.rs:34-125) ----
+
+def _refine_case():
+    # 12 gates at (5, 5) in cell (0, 0) of a 10x10 grid on a 100x100 die; six
+    # 2-pin nets inside that cell: count 6, capacity (10+10)/2 = 10, overflow
+    # 2*6-10 = 2 > 0, so the cell is hot.
+    nets = [[2 * k, 2 * k + 1] for k in range(6)]
+    ptr = np.array([0] + [2 * (k + 1) for k in range(6)], np.int32)
+    pins = np.array([g for n in nets for g in n], np.int32)
+    x = np.full(12, 5.0, np.float32)
+    y = np.full(12, 5.0, np.float32)
+    return ptr, pins, x, y
+
+
+def test_refine_hand_computed_oscillation_and_stall():
+    ptr, pins, x, y = _refine_case()
+    # Iteration 0: neighbours of (0,0) in order (dr=0: dc=1..3, dr=1..3: ...) all
+    # empty, the first minimum is (1,0) with centre (15,5): dx = clamp(10, -5, 5)
+    # = 5, so every gate lands on (10,5) = cell (1,0); iteration 1 sends them
+    # back to (5,5) via (0,0), the first neighbour of (1,0). Overflow stays 2.
+    x1, y1, it, stop, t, m = orc.refine(ptr, pins, 100.0, 100.0, x, y, max_iterations=1,
+                                        cols=10, rows=10)
+    assert it == 1 and stop == 0
+    assert np.all(x1 == 10.0) and np.all(y1 == 5.0)
+    x3, y3, it, stop, *_ = orc.refine(ptr, pins, 100.0, 100.0, x, y, max_iterations=3,
+                                      cols=10, rows=10)
+    assert it == 3 and np.all(x3 == 10.0) and np.all(y3 == 5.0)
+    # |2 - 2| < 0.01 * 2 from iteration 1 on: the tenth such iteration stops the
+    # loop (refiner.rs:60-66) after moves in iterations 0..9 (ten moves: back at 5).
+    xs, ys, it, stop, t, m = orc.refine(ptr, pins, 100.0, 100.0, x, y, cols=10, rows=10)
+    assert it == 10 and stop == 2
+    assert np.all(xs == 5.0) and np.all(ys == 5.0)
+    assert t == 2.0 and m == 2.0
+
+
+def test_refine_converged_and_no_iterations():
+    ptr, pins, x, y = _refine_case()
+    # threshold above the overflow: converged at iteration 0, nothing moves
+    xs, ys, it, stop, t, m = orc.refine(ptr, pins, 100.0, 100.0, x, y, threshold=5.0,
+                                        cols=10, rows=10)
+    assert (it, stop, t) == (0, 1, 2.0) and np.array_equal(xs, x) and np.array_equal(ys, y)
+    xs, ys, it, stop, *_ = orc.refine(ptr, pins, 100.0, 100.0, x, y, max_iterations=0,
+                                      cols=10, rows=10)
+    assert (it, stop) == (0, 0) and np.array_equal(xs, x)
+
+
+def test_refine_invariants_random():
+    # One iteration on a random placement: only gates in hot cells move, by at
+    # most move_distance per axis, toward a cell within 3 cells, inside the die.
+    rng = np.random.default_rng(5)
+    n, e, C = 3000, 3300, 32
+    sizes = rng.integers(2, 6, e)
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    pins = np.concatenate([rng.choice(n, s, replace=False) for s in sizes]).astype(np.int32)
+    x = rng.uniform(0, 1000, n).astype(np.float32)
+    y = rng.uniform(0, 1000, n).astype(np.float32)
+    cnt = orc.congestion(ptr, pins, x, y, 1000.0, 1000.0, C, C)
+    cw = np.float32(1000.0) / np.float32(C)
+    cap = (cw + cw) * np.float32(0.5)
+    col = np.minimum((x / cw).astype(np.int64), C - 1)
+    row = np.minimum((y / cw).astype(np.int64), C - 1)
+    hot = np.maximum(2 * cnt[row, col].astype(np.float32) - cap, 0) > 0
+    x1, y1, it, stop, *_ = orc.refine(ptr, pins, 1000.0, 1000.0, x, y, max_iterations=1,
+                                      cols=C, rows=C)
+    moved = (x1 != x) | (y1 != y)
+    assert it == 1 and hot.any() and not np.any(moved & ~hot)
+    # |step| <= 5 before the FP32 rounding of gx + dx (half an ulp of 1000)
+    assert np.all(np.abs(x1 - x) <= 5.0 + 3.1e-5) and np.all(np.abs(y1 - y) <= 5.0 + 3.1e-5)
+    assert x1.min() >= 0 and x1.max() <= 1000 and y1.min() >= 0 and y1.max() <= 1000
+    # every moved gate steps toward a cell whose h+v is the minimum of its 7x7
+    # neighbourhood (excluding its own cell)
+    for g in np.flatnonzero(moved)[:200]:
+        r0, c0 = row[g], col[g]
+        nb = [(r, c) for r in range(r0 - 3, r0 + 4) for c in range(c0 - 3, c0 + 4)
+              if (r, c) != (r0, c0) and 0 <= r < C and 0 <= c < C]
+        best = min(cnt[r, c] for r, c in nb)
+        tgt = [((c + 0.5) * cw, (r + 0.5) * cw) for r, c in nb if cnt[r, c] == best][0]
+        assert np.sign(x1[g] - x[g]) in (0, np.sign(np.float32(tgt[0]) - x[g]))
+        assert np.sign(y1[g] - y[g]) in (0, np.sign(np.float32(tgt[1]) - y[g]))
