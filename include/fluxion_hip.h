@@ -256,6 +256,49 @@ int fx_sort_f32(fx_ctx* ctx, int64_t n, const float* keys, uint32_t* perm);
 int fx_split_by_fiedler(fx_ctx* ctx, int32_t n, const float* fiedler, const int32_t* gates,
                         float balance_factor, int32_t* out, int32_t* left);
 
+/* ---- hierarchy (build_hierarchy, partitioner.rs:36-58; recursive_bisect,
+ * 64-100; compute_fiedler_vector, 106-185; split_by_fiedler, 189-213) ----
+ * PartitionConfig (partitioner.rs:13-33; YAML partitioning.*, default.yaml:
+ * 10-13): defaults 64, 8, 0.4, 30. balance_factor must lie in [0, 1] (above
+ * 1 the reference's split arithmetic underflows). */
+typedef struct fx_partition_config {
+  int32_t max_leaf_size;
+  int32_t min_cluster_size;
+  float balance_factor;
+  int32_t power_iterations;
+} fx_partition_config;
+void fx_partition_defaults(fx_partition_config* cfg);
+/* Node count and maximum depth of the hierarchy of num_gates gates (the
+ * tree's shape depends only on the gate count and cfg). Host only. */
+int fx_hierarchy_size(int32_t num_gates, const fx_partition_config* cfg, int32_t* num_nodes,
+                      int32_t* max_depth);
+/* Recursive spectral bisection of the circuit given by its adjacency
+ * (Circuit.adjacency, circuit.rs:68, as CSR), every level's nodes bisected
+ * together on the GPU. Nodes are numbered in the reference's pre-order.
+ * order[num_gates] receives every leaf's gate_indices (hierarchy.rs:15) in
+ * the reference's order, leaves in pre-order; node i covers
+ * order[node_start[i] .. node_start[i]+node_count[i]) (for an internal node:
+ * its gate set). node_left/right: child ids, -1 for leaves. The SpMV is the
+ * reference's FP32 sequence; mode picks the mean and norm arithmetic:
+ * FX_HIER_EXACT the reference's FP32 sums in local order (bit-identical to
+ * partitioner.rs), FX_HIER_FP64 FP64 sums in a fixed blocked order
+ * (faster; a different rounding of the same vector). A NaN Fiedler value
+ * fails with FX_E_INVALID. */
+#define FX_HIER_EXACT 0
+#define FX_HIER_FP64 1
+int fx_build_hierarchy(fx_ctx* ctx, int32_t num_gates, const int32_t* adj_ptr, const int32_t* adj,
+                       const fx_partition_config* cfg, int32_t mode, int32_t* order,
+                       int32_t* node_start,
+                       int32_t* node_count, int32_t* node_depth, int32_t* node_left,
+                       int32_t* node_right, int32_t capacity, int32_t* num_nodes);
+/* Milliseconds of the last fx_build_hierarchy's GPU work (HIP events). */
+double fx_hierarchy_last_ms(const fx_ctx* ctx);
+/* Placement regions of hapr_placement (placer.rs:43-48 root = die,
+ * place_level_recursive 111-141): region[4*i..] = center_x, center_y,
+ * width, height of node i, in FP32. Host only. */
+int fx_hierarchy_regions(int32_t num_nodes, const int32_t* node_left, const int32_t* node_right,
+                         float chip_width, float chip_height, float* region);
+
 /* ---- placement serving (continuous batching) ----
  * A server thread admits placement requests as slots and HBM budget free up,
  * keeps each request's state on the device, runs active requests' iterations
@@ -283,11 +326,43 @@ int fx_server_drain(fx_server* s);
 double fx_server_clock(const fx_server* s);
 void fx_server_destroy(fx_server* s);
 
+/* ---- on-disk formats (host only) ----
+ * fx_blif_read: blif::parse (HAPR_fluxion_v4/src/parser/blif.rs:13-122):
+ * one gate g<k> per .names/.latch/.gate line, nets of driver + readers
+ * (two or more gates), die 1000 x 1000, row height 10, design name = the
+ * file stem. The reference's net order comes from HashMap iteration and
+ * changes run to run (blif.rs:96-116); here driven nets come first in the
+ * order their names were first driven, then readers-only nets in the order
+ * their names were first read. A bare ".gate" line fails with FX_E_INVALID
+ * (the reference panics on it). */
+typedef struct fx_circuit fx_circuit;
+int fx_blif_read(const char* path, fx_circuit** out);
+void fx_circuit_free(fx_circuit* c);
+int fx_circuit_info(const fx_circuit* c, int32_t* num_gates, int32_t* num_nets,
+                    int64_t* num_pins, float* chip_width, float* chip_height);
+/* Copies the nets as CSR (net_ptr[num_nets+1], net_pins[num_pins]) and the
+ * gate types (0 combinational, 1 sequential; circuit.rs GateType). Any
+ * pointer may be NULL. */
+int fx_circuit_netlist(const fx_circuit* c, int32_t* net_ptr, int32_t* net_pins,
+                       uint8_t* gate_types);
+const char* fx_circuit_name(const fx_circuit* c);
+const char* fx_circuit_net_name(const fx_circuit* c, int32_t net);
+const char* fx_circuit_gate_name(const fx_circuit* c, int32_t gate);
+/* def::write_def (HAPR_fluxion_v4/src/parser/def.rs:12-41), byte for byte:
+ * coordinates are the FP32 products x*1000 cast as Rust's `as i64` does
+ * (truncation, NaN -> 0, saturation). gate_names NULL: g<i>. */
+int fx_def_write(const char* path, const char* design, int32_t num_gates,
+                 const char* const* gate_names, const float* x, const float* y, float chip_width,
+                 float chip_height);
+
 /* ---- host utilities (no GPU) ---- */
 /* Circuit::build_adjacency (circuit.rs:154-171) as CSR; adj NULL: count only. */
 int64_t fx_build_adjacency(int32_t num_gates, int32_t num_nets, const int32_t* net_ptr,
                            const int32_t* net_pins, int32_t* adj_ptr, int32_t* adj,
                            int64_t capacity);
+/* v_i = sin(i * 2.71828) in FP32 for i < n (partitioner.rs:145-147), with
+ * the C library's sinf (what Rust's f32::sin calls). */
+int fx_fiedler_init(int32_t n, float* v);
 /* Seeded synthetic netlists (distributions stated in netgen.cpp). With NULL
  * output buffers only the pin count is returned. */
 int64_t fx_netgen_flat(int32_t num_gates, int32_t num_nets, uint64_t seed, float die_w,

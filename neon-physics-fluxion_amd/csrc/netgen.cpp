@@ -215,3 +215,12 @@ extern "C" int64_t fx_build_adjacency(int32_t num_gates, int32_t num_nets,
   delete[] lists;
   return rc;
 }
+
+// Initial vector of compute_fiedler_vector (partitioner.rs:145-147):
+// v_i = sin(i as f32 * 2.71828) in FP32 with the host C library's sinf, the
+// function Rust's f32::sin calls on this platform.
+extern "C" int fx_fiedler_init(int32_t n, float* v) {
+  if (n < 0 || (n && !v)) return FX_E_INVALID;
+  for (int32_t i = 0; i < n; ++i) v[i] = std::sin((float)i * 2.71828f);
+  return FX_OK;
+}

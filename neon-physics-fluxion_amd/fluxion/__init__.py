@@ -5,4 +5,9 @@ from .placer import (Circuit, FlatJob, GpuContext, Placement, PlacementServer,  
                      PlacerConfig, RefineConfig, autotune_leaves, compute_fiedler_vector, dispatch_congestion,
                      dispatch_forces, fiedler_init, leaves_tile, place_leaves, place_leaves_state,
                      refine_placement, set_leaves_tile, sort_f32, sort_u64, split_by_fiedler)
+from .hierarchy import (ClusterNode, HierarchyTree, PartitionConfig, build_hierarchy,  # noqa: F401
+                        hierarchy_size)
+from .placer import hapr_placement  # noqa: F401
+from .parser import parse_blif, write_def  # noqa: F401
+from .engine import FluxionEngine  # noqa: F401
 from . import netgen  # noqa: F401

@@ -28,6 +28,9 @@ FX_FLAG_FAST_MATH = 8
 FX_LEAF_EXACT = 0
 FX_LEAF_FAST = 1
 
+FX_HIER_EXACT = 0
+FX_HIER_FP64 = 1
+
 
 class FluxionError(RuntimeError):
     """A libfluxion_hip call returned a non-zero status."""
@@ -66,6 +69,11 @@ class FxRefineConfig(C.Structure):
 class FxRefineResult(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("stop", C.c_int32), ("total_overflow", C.c_float),
                 ("max_overflow", C.c_float), ("hpwl", C.c_double), ("gate_moves", C.c_int64)]
+
+
+class FxPartitionConfig(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_int32), ("min_cluster_size", C.c_int32),
+                ("balance_factor", C.c_float), ("power_iterations", C.c_int32)]
 
 
 class FxNetlist(C.Structure):
@@ -157,7 +165,24 @@ _SIGS = {
     "fx_server_drain": (C.c_int, [_P]),
     "fx_server_clock": (_D, [_P]),
     "fx_server_destroy": (None, [_P]),
+    "fx_partition_defaults": (None, [C.POINTER(FxPartitionConfig)]),
+    "fx_hierarchy_size": (C.c_int, [_I32, C.POINTER(FxPartitionConfig), C.POINTER(_I32),
+                                    C.POINTER(_I32)]),
+    "fx_build_hierarchy": (C.c_int, [_P, _I32, _P, _P, C.POINTER(FxPartitionConfig), _I32, _P, _P,
+                                     _P, _P, _P, _P, _I32, C.POINTER(_I32)]),
+    "fx_hierarchy_last_ms": (_D, [_P]),
+    "fx_hierarchy_regions": (C.c_int, [_I32, _P, _P, _F, _F, _P]),
+    "fx_blif_read": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "fx_circuit_free": (None, [_P]),
+    "fx_circuit_info": (C.c_int, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64),
+                                  C.POINTER(_F), C.POINTER(_F)]),
+    "fx_circuit_netlist": (C.c_int, [_P, _P, _P, _P]),
+    "fx_circuit_name": (C.c_char_p, [_P]),
+    "fx_circuit_net_name": (C.c_char_p, [_P, _I32]),
+    "fx_circuit_gate_name": (C.c_char_p, [_P, _I32]),
+    "fx_def_write": (C.c_int, [C.c_char_p, C.c_char_p, _I32, _P, _P, _P, _F, _F]),
     "fx_build_adjacency": (_I64, [_I32, _I32, _P, _P, _P, _P, _I64]),
+    "fx_fiedler_init": (C.c_int, [_I32, _P]),
     "fx_netgen_flat": (_I64, [_I32, _I32, C.c_uint64, _F, _F, _P, _P, _I64, _P, _P]),
     "fx_netgen_leaves": (_I64, [_I32, C.c_uint64, _I32, _I32, _F, _P, _P, _P, _P, _P]),
 }

@@ -12,6 +12,8 @@ Reference interface (HAPR_fluxion_v4):
   * RoutingGrid overflow                src/data/grid.rs:70-91         -> FlatJob.overflow
   * compute_fiedler_vector              src/algorithm/partitioner.rs:106-185 -> compute_fiedler_vector
   * split_by_fiedler                    src/algorithm/partitioner.rs:189-213 -> split_by_fiedler
+  * hapr_placement                      src/algorithm/placer.rs:34-74  -> hapr_placement
+  * Placement::update_wirelength        src/data/placement.rs:48-80    -> Placement.update_wirelength
 
 Every call runs on the GPU through libfluxion_hip.so; errors raise FluxionError
 (the reference returns None / Result::Err or panics).
@@ -33,7 +35,7 @@ __all__ = [
     "PlacerConfig", "GpuContext", "Circuit", "Placement", "FlatJob", "PlacementServer",
     "place_leaves", "leaves_tile", "set_leaves_tile", "autotune_leaves", "dispatch_forces",
     "dispatch_congestion", "compute_fiedler_vector", "split_by_fiedler", "RefineConfig",
-    "refine_placement", "sort_f32", "sort_u64", "FluxionError",
+    "refine_placement", "sort_f32", "sort_u64", "FluxionError", "hapr_placement",
 ]
 
 
@@ -138,6 +140,11 @@ class Circuit:
     net_pins: np.ndarray
     chip_width: float = 1000.0
     chip_height: float = 1000.0
+    name: str = "circuit"
+    gate_names: list | None = None  # None: g<i> (the BLIF reader's names)
+    gate_types: np.ndarray | None = None  # 0 combinational, 1 sequential
+    net_names: list | None = None
+    row_height: float = 10.0
     _adj: tuple | None = field(default=None, repr=False)
 
     def __post_init__(self):
@@ -184,9 +191,11 @@ class Circuit:
 
 @dataclass
 class Placement:
-    """Placement (placement.rs:9-24): SoA f32 positions."""
+    """Placement (placement.rs:9-24): SoA f32 positions and quality metrics."""
     gate_x: np.ndarray
     gate_y: np.ndarray
+    total_wirelength: float = 0.0
+    max_congestion: float = 0.0
 
     @classmethod
     def new(cls, num_gates: int) -> "Placement":
@@ -195,6 +204,28 @@ class Placement:
     def set_position(self, g: int, x: float, y: float) -> None:
         self.gate_x[g] = x
         self.gate_y[g] = y
+
+    def compute_hpwl(self, circuit: "Circuit") -> float:
+        """Placement::compute_hpwl (placement.rs:48-76): per-net bounding-box
+        half perimeter in f32, accumulated in net order in f32, on the host as
+        the reference does (the GPU's FP64 reduction is FlatJob.hpwl)."""
+        deg = np.diff(circuit.net_ptr)
+        nonempty = deg >= 1
+        if not np.any(deg >= 2):
+            return 0.0
+        # Segments start at every non-empty net, so each spans exactly its net.
+        starts = circuit.net_ptr[:-1][nonempty]
+        x = self.gate_x[circuit.net_pins]
+        y = self.gate_y[circuit.net_pins]
+        w = np.maximum.reduceat(x, starts) - np.minimum.reduceat(x, starts)
+        h = np.maximum.reduceat(y, starts) - np.minimum.reduceat(y, starts)
+        multi = deg[nonempty] >= 2
+        terms = (w[multi] + h[multi]).astype(np.float32)
+        return float(np.cumsum(terms, dtype=np.float32)[-1])
+
+    def update_wirelength(self, circuit: "Circuit") -> None:
+        """Placement::update_wirelength (placement.rs:78-80)."""
+        self.total_wirelength = self.compute_hpwl(circuit)
 
 
 def _leaf_batch(leaf_ptr, leaf_gates, region, circuit: Circuit):
@@ -410,6 +441,23 @@ def dispatch_forces(ctx: GpuContext, circuit: Circuit, placement: Placement,
         placement.gate_x[:], placement.gate_y[:] = job.positions()
 
 
+def hapr_placement(ctx: GpuContext, circuit: Circuit, tree, attention=None,
+                   config: PlacerConfig | None = None, fast: bool = False) -> Placement:
+    """hapr_placement(&Circuit, &mut HierarchyTree, &AttentionMap, &PlacerConfig)
+    (placer.rs:34-74): regions level by level (root = die, grid layout per
+    level; the depth-0 spectral branch never runs because attention has no
+    level for a single root cluster), then every leaf's force-directed
+    placement in one batched launch, then the wirelength. `attention` is
+    accepted for the reference's signature and not read, as in the
+    reference's control flow."""
+    config = config or PlacerConfig()
+    leaf_ptr, leaf_gates, region = tree.leaf_batch(circuit.chip_width, circuit.chip_height)
+    placement = Placement.new(circuit.num_gates)
+    place_leaves(ctx, leaf_ptr, leaf_gates, region, circuit, placement, config, fast=fast)
+    placement.update_wirelength(circuit)
+    return placement
+
+
 def refine_placement(ctx: GpuContext, placement: Placement, circuit: Circuit,
                      config: "RefineConfig | None" = None) -> dict:
     """refine_placement(&mut Placement, &Circuit, &RefineConfig) (refiner.rs:34-39):
@@ -436,9 +484,12 @@ def dispatch_congestion(ctx: GpuContext, cols: int, rows: int, placement: Placem
 
 
 def fiedler_init(n: int) -> np.ndarray:
-    """partitioner.rs:145-147: v_i = sin(i * 2.71828) in f32 (computed on the host)."""
-    i = np.arange(n, dtype=np.float32)
-    return np.sin(i * np.float32(2.71828)).astype(np.float32)
+    """partitioner.rs:145-147: v_i = sin(i * 2.71828) in f32, with the C
+    library's sinf as Rust's f32::sin (numpy's float32 sin rounds
+    differently for a few percent of arguments)."""
+    v = np.zeros(max(int(n), 1), np.float32)
+    check(lib.fx_fiedler_init(int(n), ptr(v)))
+    return v[:n]
 
 
 def compute_fiedler_vector(ctx: GpuContext, circuit: Circuit, gate_indices, num_iterations=30,

@@ -1111,3 +1111,245 @@ ORC_API void orc_power_iter_f64(int32_t n, const int32_t* gates, int32_t num_gat
   }
   std::memcpy(out, v.data(), sizeof(double) * n);
 }
+
+// ---------------------------------------------------------------------------
+// Hierarchy: build_hierarchy -> recursive_bisect -> compute_fiedler_vector ->
+// split_by_fiedler (HAPR_fluxion_v4/src/algorithm/partitioner.rs:36-58,
+// 64-100, 106-185, 189-213).
+//
+// mode 0 follows the reference's arithmetic: FP32 sums accumulated in local
+// order for the mean (168) and the norms (149, 174), norm compared with
+// 1e-10f.
+// mode 2 sums the mean and the squared norms in FP64 in a fixed blocked
+// order: the local order is cut into 256-element chunks, each chunk is
+// reduced by a stride-halving pairwise tree (slot k += slot k+h for
+// h = 128, 64, ..., 1; missing slots are +0.0); the chunk results P_c fold
+// into 256 slots Q_k = P_k + P_{k+256} + ... (index order) which the same
+// tree reduces. mean = (float)(sum/n),
+// v = (float)(lv/norm) in double, norm compared with 1e-10.
+//
+// Node ids are assigned in the reference's pre-order (65-66, 91-92). The
+// output order[] lists every leaf's gates, leaves in pre-order; every node
+// covers order[start, start+count). A leaf's slice is its gate_indices in
+// the reference's order (the stable sort of 196-202, left part then right
+// part); an internal node's slice holds its gate set. Returns the number of
+// nodes, -1 if `capacity` is too small, -2 on a NaN Fiedler value (the
+// reference's sort_by(partial_cmp) has no order for NaN).
+// ---------------------------------------------------------------------------
+namespace {
+
+double chunk_tree_sum(const double* x, int64_t len) {
+  double s[256];
+  for (int k = 0; k < 256; ++k) s[k] = k < len ? x[k] : 0.0;
+  for (int h = 128; h > 0; h >>= 1)
+    for (int k = 0; k < h; ++k) s[k] += s[k + h];
+  return s[0];
+}
+
+double blocked_sum(const std::vector<double>& x) {
+  const int64_t n = (int64_t)x.size();
+  const int64_t m = (n + 255) / 256;
+  std::vector<double> part(m);
+  for (int64_t c = 0; c < m; ++c)
+    part[c] = chunk_tree_sum(x.data() + 256 * c, std::min<int64_t>(n - 256 * c, 256));
+  double q[256];
+  for (int64_t k = 0; k < 256; ++k) {
+    q[k] = 0.0;
+    if (k < m) {
+      q[k] = part[k];
+      for (int64_t i = k + 256; i < m; i += 256) q[k] += part[i];
+    }
+  }
+  return chunk_tree_sum(q, 256);
+}
+
+struct HierBuilder {
+  const int32_t* adj_ptr;
+  const int32_t* adj;
+  int32_t max_leaf, min_cluster, iters, mode;
+  float bf;
+  std::vector<int32_t> g2l;  // scratch: global -> local, -1 outside
+  std::vector<int32_t> order;
+  struct Node { int32_t start, count, depth, left, right; };
+  std::vector<Node> nodes;
+  bool nan = false;
+
+  // compute_fiedler_vector (partitioner.rs:106-185).
+  std::vector<float> fiedler(const std::vector<int32_t>& gates) {
+    const int32_t n = (int32_t)gates.size();
+    if (n <= 1) return std::vector<float>(n, 0.0f);
+    for (int32_t i = 0; i < n; ++i) g2l[gates[i]] = i;
+    std::vector<float> degree(n, 0.0f);
+    std::vector<std::pair<int32_t, int32_t>> pairs;
+    for (int32_t li = 0; li < n; ++li) {
+      const int32_t g = gates[li];
+      for (int32_t k = adj_ptr[g]; k < adj_ptr[g + 1]; ++k) {
+        const int32_t lj = g2l[adj[k]];
+        if (lj < 0) continue;
+        pairs.push_back({li, lj});
+        degree[li] += 1.0f;
+      }
+    }
+    for (int32_t i = 0; i < n; ++i) g2l[gates[i]] = -1;
+    std::vector<float> v(n), lv(n);
+    for (int32_t i = 0; i < n; ++i) v[i] = std::sin((float)i * 2.71828f);
+    std::vector<double> tmp(n);
+    auto normalise = [&](std::vector<float>& a) {
+      if (mode == 0) {
+        float s = 0.0f;
+        for (float t : a) s += t * t;
+        const float nrm = std::sqrt(s);
+        if (nrm > 1e-10f)
+          for (float& t : a) t /= nrm;
+      } else {
+        for (int32_t i = 0; i < n; ++i) tmp[i] = (double)a[i] * (double)a[i];
+        const double nrm = std::sqrt(blocked_sum(tmp));
+        if (nrm > 1e-10)
+          for (float& t : a) t = (float)((double)t / nrm);
+      }
+    };
+    normalise(v);
+    for (int32_t it = 0; it < iters; ++it) {
+      for (int32_t i = 0; i < n; ++i) lv[i] = degree[i] * v[i];
+      for (auto& pr : pairs) lv[pr.first] -= v[pr.second];
+      if (mode == 0) {
+        float s = 0.0f;
+        for (float t : lv) s += t;
+        const float mean = s / (float)n;
+        for (float& t : lv) t -= mean;
+      } else {
+        for (int32_t i = 0; i < n; ++i) tmp[i] = (double)lv[i];
+        const float mean = (float)(blocked_sum(tmp) / (double)n);
+        for (float& t : lv) t -= mean;
+      }
+      normalise(lv);
+      v.swap(lv);
+    }
+    return v;
+  }
+
+  // recursive_bisect (partitioner.rs:64-100); gates are written to
+  // order[start..] as the leaves are reached.
+  int32_t bisect(const std::vector<int32_t>& gates, int32_t depth, int32_t start) {
+    const int32_t id = (int32_t)nodes.size();
+    const int32_t n = (int32_t)gates.size();
+    nodes.push_back({start, n, depth, -1, -1});
+    auto leaf = [&]() {
+      std::copy(gates.begin(), gates.end(), order.begin() + start);
+      return id;
+    };
+    if (n <= max_leaf || n <= min_cluster) return leaf();
+    std::vector<float> f = fiedler(gates);
+    // split_by_fiedler: stable sort by value, -0.0 == +0.0.
+    std::vector<int32_t> idx(n);
+    for (int32_t i = 0; i < n; ++i) {
+      idx[i] = i;
+      if (f[i] != f[i]) nan = true;
+    }
+    if (nan) return leaf();
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return f[a] < f[b]; });
+    const int32_t min_size = (int32_t)std::ceil((float)n * bf);
+    int32_t split = std::min(std::max(n / 2, min_size), n - min_size);
+    if (split <= 0 || split >= n) return leaf();
+    std::vector<int32_t> lg(split), rg(n - split);
+    for (int32_t i = 0; i < split; ++i) lg[i] = gates[idx[i]];
+    for (int32_t i = split; i < n; ++i) rg[i - split] = gates[idx[i]];
+    const int32_t l = bisect(lg, depth + 1, start);
+    const int32_t r = bisect(rg, depth + 1, start + split);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    return id;
+  }
+};
+
+}  // namespace
+
+ORC_API int32_t orc_build_hierarchy(int32_t num_gates, const int32_t* adj_ptr, const int32_t* adj,
+                                    int32_t max_leaf, int32_t min_cluster, float balance_factor,
+                                    int32_t iters, int32_t mode, int32_t* order,
+                                    int32_t* node_start, int32_t* node_count, int32_t* node_depth,
+                                    int32_t* node_left, int32_t* node_right, int32_t capacity) {
+  HierBuilder b{adj_ptr, adj, max_leaf, min_cluster, iters, mode, balance_factor};
+  b.g2l.assign(num_gates, -1);
+  b.order.assign(num_gates, -1);
+  std::vector<int32_t> all(num_gates);
+  for (int32_t g = 0; g < num_gates; ++g) all[g] = g;
+  b.bisect(all, 0, 0);
+  if (b.nan) return -2;
+  if ((int64_t)b.nodes.size() > capacity) return -1;
+  std::memcpy(order, b.order.data(), sizeof(int32_t) * num_gates);
+  for (size_t i = 0; i < b.nodes.size(); ++i) {
+    node_start[i] = b.nodes[i].start;
+    node_count[i] = b.nodes[i].count;
+    node_depth[i] = b.nodes[i].depth;
+    node_left[i] = b.nodes[i].left;
+    node_right[i] = b.nodes[i].right;
+  }
+  return (int32_t)b.nodes.size();
+}
+
+// compute_fiedler_vector alone (partitioner.rs:106-185) for the gates of
+// one node in their local order, mode as orc_build_hierarchy.
+ORC_API void orc_fiedler_vector(int32_t n, const int32_t* gates, int32_t num_gates,
+                                const int32_t* adj_ptr, const int32_t* adj, int32_t iters,
+                                int32_t mode, float* out) {
+  HierBuilder b{adj_ptr, adj, 0, 0, iters, mode, 0.0f};
+  b.g2l.assign(num_gates, -1);
+  std::vector<int32_t> g(gates, gates + n);
+  std::vector<float> v = b.fiedler(g);
+  std::memcpy(out, v.data(), sizeof(float) * n);
+}
+
+// Regions of hapr_placement (placer.rs:43-64): root = die, then for each
+// depth 1..max_depth place_level_recursive (111-141) walks from the root and
+// lays out the children of every node at depth-1 on a grid. The depth-0
+// spectral branch (52-58) does not run (no attention level for one
+// cluster). region[4*i..]: center_x, center_y, width, height.
+namespace {
+struct RegionTree {
+  const int32_t* depth;
+  const int32_t* left;
+  const int32_t* right;
+  float* region;
+  void place_level(int32_t node, int32_t target) {
+    const bool has_children = left[node] >= 0;
+    if (depth[node] + 1 == target && has_children) {
+      const int32_t kids[2] = {left[node], right[node]};
+      const size_t n = 2;
+      float* P = region + 4 * node;
+      const float half_w = P[2] / 2.0f, half_h = P[3] / 2.0f;
+      const float origin_x = P[0] - half_w, origin_y = P[1] - half_h;
+      const size_t cols = (size_t)std::ceil(std::sqrt((float)n));
+      const size_t rows = (n + cols - 1) / cols;
+      for (size_t i = 0; i < n; ++i) {
+        const size_t col = i % cols, row = i / cols;
+        const float cw = P[2] / (float)cols;
+        const float ch = P[3] / (float)rows;
+        const float cx = origin_x + ((float)col + 0.5f) * cw;
+        const float cy = origin_y + ((float)row + 0.5f) * ch;
+        float* C = region + 4 * kids[i];
+        C[0] = cx;
+        C[1] = cy;
+        C[2] = cw;
+        C[3] = ch;
+      }
+    } else if (has_children) {
+      place_level(left[node], target);
+      place_level(right[node], target);
+    }
+  }
+};
+}  // namespace
+
+ORC_API void orc_hapr_regions(int32_t num_nodes, const int32_t* depth, const int32_t* left,
+                              const int32_t* right, float chip_w, float chip_h, float* region) {
+  if (num_nodes <= 0) return;
+  int32_t max_depth = 0;
+  for (int32_t i = 0; i < num_nodes; ++i) max_depth = std::max(max_depth, depth[i]);
+  region[0] = chip_w / 2.0f;
+  region[1] = chip_h / 2.0f;
+  region[2] = chip_w;
+  region[3] = chip_h;
+  RegionTree t{depth, left, right, region};
+  for (int32_t d = 1; d <= max_depth; ++d) t.place_level(0, d);
+}

@@ -39,6 +39,10 @@ _SIGS = {
                                 _I32, _I32, _I32, _P, _P, _P, _P, _P, _P]),
     "orc_power_iter_f32": (None, [_I32, _P, _I32, _P, _P, _P, _I32, _I32, _P]),
     "orc_power_iter_f64": (None, [_I32, _P, _I32, _P, _P, _P, _I32, _P]),
+    "orc_build_hierarchy": (_I32, [_I32, _P, _P, _I32, _I32, _F, _I32, _I32, _P, _P, _P, _P, _P,
+                                   _P, _I32]),
+    "orc_fiedler_vector": (None, [_I32, _P, _I32, _P, _P, _I32, _I32, _P]),
+    "orc_hapr_regions": (None, [_I32, _P, _P, _P, _F, _F, _P]),
 }
 
 
@@ -250,3 +254,39 @@ def power_iter(gates, n_total, aptr, adj, v0, iters, mode=0, dtype=np.float32):
         out = np.zeros(n, np.float64)
         lib.orc_power_iter_f64(n, p(gates), n_total, p(i32(aptr)), p(adj), p(f64(v0)), iters, p(out))
     return out
+
+
+def fiedler_vector(gates, n_total, aptr, adj, iters=30, mode=0):
+    """compute_fiedler_vector of one node (partitioner.rs:106-185); mode 0 is
+    the reference's FP32 arithmetic, mode 2 the blocked FP64 reductions."""
+    gates = i32(gates)
+    n = gates.shape[0]
+    adj = i32(adj) if len(adj) else np.zeros(1, np.int32)
+    out = np.zeros(n, np.float32)
+    lib.orc_fiedler_vector(n, p(gates), n_total, p(i32(aptr)), p(adj), iters, mode, p(out))
+    return out
+
+
+def build_hierarchy(num_gates, aptr, adj, max_leaf=64, min_cluster=8, balance=0.4, iters=30,
+                    mode=0):
+    """build_hierarchy (partitioner.rs:36-100). Returns (order, nodes) with
+    nodes a dict of int32 arrays start/count/depth/left/right in pre-order."""
+    adj = i32(adj) if len(adj) else np.zeros(1, np.int32)
+    cap = 4 * max(num_gates, 1) + 8
+    order = np.zeros(max(num_gates, 1), np.int32)
+    arrs = [np.zeros(cap, np.int32) for _ in range(5)]
+    nn = lib.orc_build_hierarchy(num_gates, p(i32(aptr)), p(adj), max_leaf, min_cluster, balance,
+                                 iters, mode, p(order), *[p(a) for a in arrs], cap)
+    if nn < 0:
+        raise RuntimeError(f"orc_build_hierarchy failed ({nn})")
+    names = ("start", "count", "depth", "left", "right")
+    return order[:num_gates], {k: a[:nn].copy() for k, a in zip(names, arrs)}
+
+
+def hapr_regions(nodes, chip_w, chip_h):
+    """Regions of hapr_placement (placer.rs:43-64, 111-141): [num_nodes, 4]."""
+    nn = len(nodes["depth"])
+    out = np.zeros((max(nn, 1), 4), np.float32)
+    lib.orc_hapr_regions(nn, p(i32(nodes["depth"])), p(i32(nodes["left"])), p(i32(nodes["right"])),
+                         chip_w, chip_h, p(out))
+    return out[:nn]
