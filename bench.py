@@ -277,6 +277,95 @@ def flat2m_workload(fxn, ctx, iters=20, fiedler_steps=200):
     return out
 
 
+def hierarchy_pipeline(fxn, ctx, cpu=True):
+    """Recursive spectral bisection (partitioner.rs:36-213) at 200k and 2M
+    gates, and the whole BLIF -> hierarchy -> hapr_placement ->
+    refine_placement -> DEF pipeline (main.rs:27-121) at 200k gates, with the
+    oracle's single-thread CPU pipeline timed on the same circuit."""
+    import tempfile
+
+    import numpy as np
+    out = {}
+    for name in ("flat_200k", "flat_2m"):
+        N, E, off = WORKLOADS[name][:3]
+        circ, _, _ = fxn.netgen.flat(N, E, fxn.netgen.SEED_BASE + off)
+        circ.build_adjacency()
+        rec = {}
+        for exact in ((True, False) if N <= 200000 else (False,)):
+            fxn.build_hierarchy(ctx, circ, exact=exact)  # warm
+            t0 = time.perf_counter()
+            tree = fxn.build_hierarchy(ctx, circ, exact=exact)
+            rec["exact" if exact else "fp64"] = {
+                "gpu_ms": tree.build_ms, "call_ms": (time.perf_counter() - t0) * 1e3,
+                "nodes": tree.num_nodes, "max_depth": tree.max_depth}
+        out[name] = rec
+    # Pipeline on a synthetic BLIF of the 200k-gate netlist.
+    N, E, off = WORKLOADS["flat_200k"][:3]
+    circ, _, _ = fxn.netgen.flat(N, E, fxn.netgen.SEED_BASE + off)
+    with tempfile.TemporaryDirectory() as td:
+        blif = os.path.join(td, "flat200k.blif")
+        drives = [[] for _ in range(N)]
+        reads = [[] for _ in range(N)]
+        for e in range(circ.num_nets):
+            pins = circ.net_pins[circ.net_ptr[e]:circ.net_ptr[e + 1]]
+            drives[pins[0]].append(e)
+            for g in pins[1:]:
+                reads[g].append(e)
+        with open(blif, "w") as f:
+            f.write(".model flat200k\n")
+            for g in range(N):
+                f.write(".gate CELL " + " ".join([f"I{k}=e{e}" for k, e in enumerate(reads[g])] +
+                                                 [f"O=e{e}" for e in drives[g]]) + "\n")
+            f.write(".end\n")
+        cfg = os.path.join(ROOT, "tests", "golden", "default.yaml")
+        eng = fxn.FluxionEngine(cfg, ctx=ctx)
+        eng.place(blif, os.path.join(td, "w.def"))  # warm
+        t0 = time.perf_counter()
+        eng.place(blif, os.path.join(td, "placed.def"))
+        total = time.perf_counter() - t0
+        st = {k: v for k, v in eng.stats.items() if k != "refine"}
+        st["refine_iterations"] = eng.stats["refine"]["iterations"]
+        st["total_s"] = total
+        out["pipeline_200k"] = st
+        if cpu:
+            # The oracle's pipeline, single thread: adjacency, hierarchy and
+            # leaf placement in full; refinement for one iteration (one
+            # iteration of refiner.rs costs tens of seconds here), scaled by
+            # the GPU run's iteration count.
+            sys.path.insert(0, ROOT)
+            from oracle import oracle as orc
+            c2 = fxn.parse_blif(blif)
+            pc, pl, rc = eng.partition_config(), eng.placer_config(), eng.refine_config()
+            t0 = time.perf_counter()
+            aptr, adj = orc.build_adjacency(N, c2.net_ptr, c2.net_pins)
+            t1 = time.perf_counter()
+            order, nodes = orc.build_hierarchy(N, aptr, adj, pc.max_leaf_size, pc.min_cluster_size,
+                                               pc.balance_factor, pc.power_iterations, mode=0)
+            t2 = time.perf_counter()
+            reg = orc.hapr_regions(nodes, c2.chip_width, c2.chip_height)
+            lv = np.flatnonzero(nodes["left"] < 0)
+            lp = np.zeros(len(lv) + 1, np.int32)
+            lp[1:] = np.cumsum(nodes["count"][lv])
+            x, y, *_ = orc.place_leaves(lp, order, reg[lv], N, aptr, adj, pl.attract_weight,
+                                        pl.repel_weight, pl.damping, pl.dt, pl.force_iterations)
+            t3 = time.perf_counter()
+            orc.refine(c2.net_ptr, c2.net_pins, c2.chip_width, c2.chip_height, x, y, 1,
+                       rc.move_distance, rc.overflow_threshold, rc.grid_cols, rc.grid_rows)
+            t4 = time.perf_counter()
+            tree = fxn.build_hierarchy(ctx, c2, pc)
+            gp = fxn.hapr_placement(ctx, c2, tree, None, pl)
+            its = max(1, st["refine_iterations"])
+            out["pipeline_200k_cpu"] = {
+                "kind": "port", "cores": 1, "adjacency_s": t1 - t0, "hierarchy_s": t2 - t1,
+                "placement_s": t3 - t2, "refine_s_one_iteration": t4 - t3,
+                "total_s_est": (t3 - t0) + (t4 - t3) * its,
+                "sample": f"adjacency, hierarchy and leaf placement in full; refinement timed "
+                          f"for 1 of the GPU run's {its} iterations and scaled",
+                "same_placement_as_gpu": bool(np.array_equal(x, gp.gate_x) and
+                                              np.array_equal(y, gp.gate_y))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -443,6 +532,7 @@ def main():
         out["refine"] = ref
         out["leaves_4096"] = leaves_workload(fxn, ctx)
         out["flat_2m"] = flat2m_workload(fxn, ctx)
+        out["hierarchy"] = hierarchy_pipeline(fxn, ctx, cpu=not args.no_cpu_baseline)
     if args.serve_requests > 0:
         out["serving"] = serving(fxn, ctx, args.serve_requests, args.serve_iters)
     if not args.no_cpu_baseline:

@@ -278,57 +278,68 @@ __global__ __launch_bounds__(kB) void grid_center(const Seg* __restrict__ segs,
 
 // ---- exact mode: FP32 sums in local order, one lane per node ----
 constexpr int kStage = 2048;
+constexpr int kPad = 32;  // floats readable past the end of a serial_sum input
 
-// s += x_i (kSquare: s += (x_i - mean) * (x_i - mean)) for i = 0..len-1, in
-// order, over values in LDS.
-template <bool kSquare>
-__device__ __forceinline__ float serial_sum(const float* b, int len, float mean, float s) {
+// s += b_i for i = 0..len-1, in order, over values in LDS (b must hold
+// len + kPad floats). One lane runs
+// it: each add waits for the previous one, so the loop body is nothing but
+// the adds and the LDS reads that feed them.
+__device__ __forceinline__ void add32(float& s, const float4 (&q)[8]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    s += q[u].x;
+    s += q[u].y;
+    s += q[u].z;
+    s += q[u].w;
+  }
+}
+
+__device__ __forceinline__ float serial_sum(const float* b, int len, float s) {
   const float4* b4 = reinterpret_cast<const float4*>(b);
   int i = 0;
-  for (; i + 16 <= len; i += 16) {
-    float x[16];
+  // Two register sets alternate: while one set's 32 values are added, the
+  // other set's LDS reads are in flight (no register copies in the loop).
+  float4 qa[8], qb[8];
+  if (len >= 64) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float4 q = b4[(i >> 2) + u];
-      x[4 * u] = q.x;
-      x[4 * u + 1] = q.y;
-      x[4 * u + 2] = q.z;
-      x[4 * u + 3] = q.w;
-    }
+    for (int u = 0; u < 8; ++u) qa[u] = b4[u];
+  }
+  for (; i + 64 <= len; i += 64) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      float y = x[u];
-      if (kSquare) {
-        y = y - mean;
-        y = y * y;
-      }
-      s += y;
-    }
+    for (int u = 0; u < 8; ++u) qb[u] = b4[((i + 32) >> 2) + u];
+    add32(s, qa);
+    // Unconditional: b holds kPad floats past its last value, and values
+    // read past len are never added.
+#pragma unroll
+    for (int u = 0; u < 8; ++u) qa[u] = b4[((i + 64) >> 2) + u];
+    add32(s, qb);
   }
-  for (; i < len; ++i) {
-    float y = b[i];
-    if (kSquare) {
-      y = y - mean;
-      y = y * y;
-    }
-    s += y;
-  }
+  for (; i < len; ++i) s += b[i];
   return s;
 }
 
-// The same over n values in global memory: the workgroup stages kStage
-// values at a time in LDS (double-buffered) while lane 0 accumulates.
-// Result valid in thread 0.
+// (x - mean) * (x - mean) in FP32, the reference's square of a centred
+// entry (partitioner.rs:169-174); x itself when !kSquare.
+template <bool kSquare>
+__device__ __forceinline__ float term(float x, float mean) {
+  if (!kSquare) return x;
+  const float t = x - mean;
+  return t * t;
+}
+
+// The same over the terms of n values in global memory: the workgroup
+// computes and stages kStage terms at a time in LDS (double-buffered)
+// while lane 0 accumulates. Result valid in thread 0.
 template <bool kSquare>
 __device__ float chain_sum(const float* __restrict__ src, int n, float mean,
-                           float (*buf)[kStage]) {
+                           float (*buf)[kStage + kPad]) {
   const int t = threadIdx.x;
   const int nch = (n + kStage - 1) / kStage;
   constexpr int kPer = kStage / kB;
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int i = k * kB + t;
-    if (i < n) buf[0][i] = src[i];
+    if (i < n) buf[0][i] = term<kSquare>(src[i], mean);
   }
   __syncthreads();
   float s = 0.0f;
@@ -342,12 +353,12 @@ __device__ float chain_sum(const float* __restrict__ src, int n, float mean,
         r[k] = i < n ? src[i] : 0.0f;
       }
     }
-    if (t == 0) s = serial_sum<kSquare>(buf[c & 1], min(kStage, n - c * kStage), mean, s);
+    if (t == 0) s = serial_sum(buf[c & 1], min(kStage, n - c * kStage), s);
     if (c + 1 < nch) {
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
         const int i = k * kB + t;
-        if (next + i < n) buf[(c + 1) & 1][i] = r[k];
+        if (next + i < n) buf[(c + 1) & 1][i] = term<kSquare>(r[k], mean);
       }
     }
     __syncthreads();
@@ -359,7 +370,7 @@ __device__ float chain_sum(const float* __restrict__ src, int n, float mean,
 __global__ __launch_bounds__(kB) void chain_kernel(const Seg* __restrict__ segs,
                                                    const float* __restrict__ lv, int center,
                                                    float2* __restrict__ stats) {
-  __shared__ __attribute__((aligned(16))) float buf[2][kStage];
+  __shared__ __attribute__((aligned(16))) float buf[2][kStage + kPad];
   __shared__ float sh_mean;
   const Seg s = segs[blockIdx.x];
   const float* src = lv + s.start;
@@ -454,8 +465,8 @@ __global__ __launch_bounds__(kB) void wg_bisect(const Seg* __restrict__ segs,
                                                 const int* __restrict__ col,
                                                 const float* __restrict__ sinit, int iters,
                                                 int* __restrict__ perm, int* nan_flag) {
-  __shared__ float v[kWgMax];
-  __shared__ __attribute__((aligned(16))) float lv[kWgMax];
+  __shared__ __attribute__((aligned(16))) float v[kWgMax + kPad];
+  __shared__ __attribute__((aligned(16))) float lv[kWgMax + kPad];
   __shared__ uint64_t kbuf[kWgMax];  // reduction slots, then sort keys
   __shared__ double out;
   __shared__ float outf;
@@ -495,12 +506,14 @@ __global__ __launch_bounds__(kB) void wg_bisect(const Seg* __restrict__ segs,
     } else {
       float mean = 0.0f;
       if (center) {
-        if (t == 0) outf = serial_sum<false>(lv, n, 0.0f, 0.0f) / (float)n;
+        if (t == 0) outf = serial_sum(lv, n, 0.0f) / (float)n;
         __syncthreads();
         mean = outf;
-        __syncthreads();
       }
-      if (t == 0) outf = sqrtf(serial_sum<true>(lv, n, mean, 0.0f));
+      // Squares in parallel (v is free until the normalisation below).
+      for (int i = t; i < n; i += kB) v[i] = term<true>(lv[i], mean);
+      __syncthreads();
+      if (t == 0) outf = sqrtf(serial_sum(v, n, 0.0f));
       __syncthreads();
       const float nrm = outf;
 #pragma unroll
