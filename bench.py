@@ -298,6 +298,14 @@ def hierarchy_pipeline(fxn, ctx, cpu=True):
             rec["exact" if exact else "fp64"] = {
                 "gpu_ms": tree.build_ms, "call_ms": (time.perf_counter() - t0) * 1e3,
                 "nodes": tree.num_nodes, "max_depth": tree.max_depth}
+        # Attention scoring of every level (attention.rs:33-88), top_k 10.
+        fxn.compute_all_attention(ctx, tree, circ)
+        t0 = time.perf_counter()
+        amap = fxn.compute_all_attention(ctx, tree, circ)
+        rec["attention"] = {"gpu_ms": amap.build_ms, "call_ms": (time.perf_counter() - t0) * 1e3,
+                            "levels": len(amap.levels),
+                            "clusters_deepest": int(amap.levels[-1].scores.shape[0]),
+                            "nonzero_scores": int(sum(lv.scores.nnz for lv in amap.levels))}
         out[name] = rec
     # Pipeline on a synthetic BLIF of the 200k-gate netlist.
     N, E, off = WORKLOADS["flat_200k"][:3]

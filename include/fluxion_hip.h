@@ -299,6 +299,34 @@ double fx_hierarchy_last_ms(const fx_ctx* ctx);
 int fx_hierarchy_regions(int32_t num_nodes, const int32_t* node_left, const int32_t* node_right,
                          float chip_width, float chip_height, float* region);
 
+/* ---- attention scoring (compute_all_attention, attention.rs:33-48;
+ * compute_level_attention, 50-88) ----
+ * For every depth with at least two clusters (the hierarchy's nodes of that
+ * depth, in pre-order): shared[i][j] = number of nets touching both
+ * clusters, total[i] = number of nets touching cluster i,
+ * score[i][j] = (float)shared / (float)total[i] for i != j, and top_k[i] =
+ * the j of the k largest scores (ties: ascending j), zero scores dropped.
+ * The node table is fx_build_hierarchy's (order, node_start/count/depth).
+ * Scores are returned as CSR over the non-zero off-diagonal entries (the
+ * reference's dense n x n matrix holds zeros everywhere else). */
+typedef struct fx_attention fx_attention;
+int fx_attention_build(fx_ctx* ctx, const fx_netlist* net, int32_t num_nodes, const int32_t* order,
+                       const int32_t* node_start, const int32_t* node_count,
+                       const int32_t* node_depth, int32_t top_k, fx_attention** out);
+int32_t fx_attention_num_levels(const fx_attention* a);
+/* level_index: the depth (AttentionLevel.level_index); nnz: off-diagonal
+ * non-zeros. */
+int fx_attention_level_info(const fx_attention* a, int32_t level, int32_t* level_index,
+                            int32_t* num_clusters, int64_t* nnz);
+/* cluster_node[n]: hierarchy node of each cluster; total[n]; row_ptr[n+1];
+ * col/shared/score[nnz]; top[n*top_k] (-1 padded). Any pointer may be NULL. */
+int fx_attention_level_get(const fx_attention* a, int32_t level, int32_t* cluster_node,
+                           int32_t* total, int64_t* row_ptr, int32_t* col, int32_t* shared,
+                           float* score, int32_t* top);
+void fx_attention_free(fx_attention* a);
+/* Milliseconds of the last fx_attention_build's GPU work (HIP events). */
+double fx_attention_last_ms(const fx_ctx* ctx);
+
 /* ---- placement serving (continuous batching) ----
  * A server thread admits placement requests as slots and HBM budget free up,
  * keeps each request's state on the device, runs active requests' iterations

@@ -1,0 +1,462 @@
+// Attention scoring: compute_all_attention -> compute_level_attention
+// (HAPR_fluxion_v4/src/algorithm/attention.rs:33-99).
+//
+// For every depth with at least two clusters (40-43), the clusters are the
+// hierarchy nodes of exactly that depth in pre-order (hierarchy.rs:140-171),
+// N_i the set of nets touching cluster i (101-110), and
+//   score[i][j] = |N_i ∩ N_j| / |N_i|   (FP32 division of the two counts),
+// 0 on the diagonal and for clusters that touch no net; top_k[i] lists the
+// j != i of the k largest scores, ties in ascending j (stable sort, 74-84),
+// zero scores dropped.
+//
+// |N_i ∩ N_j| for all pairs at once, by sorting instead of set
+// intersections: each pin gives a (net, cluster) key; sorting and keeping
+// the first of equal keys gives every net's distinct clusters; every net
+// with k distinct clusters then emits its k^2 ordered (cluster, cluster)
+// pairs; sorting those and counting runs gives shared[i][j] for every pair
+// with a common net, and shared[i][i] = |N_i|. Scores and the top-k lists
+// come from the sorted runs (a stable sort by descending score per row).
+// All counts are integers; the scores are the reference's single FP32
+// division per entry.
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "internal.h"
+
+struct fx_attention {
+  struct Level {
+    int32_t depth = 0;
+    std::vector<int32_t> nodes;    // cluster -> hierarchy node id
+    std::vector<int32_t> total;    // |N_i|
+    std::vector<int64_t> row_ptr;  // CSR of the non-zero off-diagonal scores
+    std::vector<int32_t> col, shared;
+    std::vector<float> score;
+    std::vector<int32_t> top;      // [n * k], -1 padded
+  };
+  int32_t top_k = 10;
+  std::vector<Level> levels;
+};
+
+namespace fx {
+namespace {
+
+constexpr int kB = 256;
+constexpr uint64_t kNone = ~0ull;
+
+__device__ __forceinline__ uint32_t order_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void pin_net_kernel(int32_t E, const int32_t* __restrict__ net_ptr,
+                               int32_t* __restrict__ pin_net) {
+  const int32_t e = blockIdx.x * kB + threadIdx.x;
+  if (e >= E) return;
+  for (int32_t p = net_ptr[e]; p < net_ptr[e + 1]; ++p) pin_net[p] = e;
+}
+
+// cl[g] = index of the depth's cluster holding gate g (range table, one
+// block per cluster); -1 elsewhere (cleared before).
+__global__ void label_kernel(const int2* __restrict__ ranges, const int32_t* __restrict__ order,
+                             int32_t* __restrict__ cl) {
+  const int2 r = ranges[blockIdx.x];
+  for (int32_t i = threadIdx.x; i < r.y; i += kB) cl[order[r.x + i]] = blockIdx.x;
+}
+
+__global__ void pin_keys_kernel(int64_t P, const int32_t* __restrict__ pins,
+                                const int32_t* __restrict__ pin_net,
+                                const int32_t* __restrict__ cl, int cbits,
+                                uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (p >= P) return;
+  const int32_t c = cl[pins[p]];
+  keys[p] = c >= 0 ? ((uint64_t)pin_net[p] << cbits) | (uint32_t)c : kNone;
+  vals[p] = (uint32_t)p;
+}
+
+// flag[q] = 1 for the first of a run of equal keys (kNone excluded).
+__global__ void head_flags(int64_t n, const uint64_t* __restrict__ keys, int32_t* __restrict__ flag) {
+  const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (q >= n) return;
+  const uint64_t k = keys[q];
+  flag[q] = k != kNone && (q == 0 || keys[q - 1] != k);
+}
+
+__global__ void compact_keys(int64_t n, const uint64_t* __restrict__ keys,
+                             const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                             uint64_t* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (q < n && flag[q]) out[pos[q]] = keys[q];
+}
+
+// Per distinct (net, cluster): the net's first entry and its cluster count.
+__global__ void net_runs(int64_t U, const uint64_t* __restrict__ uec, int cbits,
+                         int32_t* __restrict__ kstart, int32_t* __restrict__ kcnt) {
+  const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (q >= U) return;
+  const int32_t e = (int32_t)(uec[q] >> cbits);
+  if (q == 0 || (int32_t)(uec[q - 1] >> cbits) != e) kstart[e] = (int32_t)q;
+  atomicAdd(&kcnt[e], 1);
+}
+
+struct I64x2d {
+  int64_t a, b;
+};
+
+__global__ void pair_counts(int64_t U, const uint64_t* __restrict__ uec, int cbits,
+                            const int32_t* __restrict__ kcnt, I64x2d* __restrict__ cnt) {
+  const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (q >= U) return;
+  cnt[q] = {kcnt[uec[q] >> cbits], 0};
+}
+
+__global__ void emit_pairs(int64_t U, const uint64_t* __restrict__ uec, int cbits,
+                           const int32_t* __restrict__ kstart, const int32_t* __restrict__ kcnt,
+                           const I64x2d* __restrict__ off, uint64_t* __restrict__ pk,
+                           uint32_t* __restrict__ pv) {
+  const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (q >= U) return;
+  const uint64_t mask = (1ull << cbits) - 1;
+  const int32_t e = (int32_t)(uec[q] >> cbits);
+  const uint64_t ci = uec[q] & mask;
+  const int32_t s = kstart[e], k = kcnt[e];
+  int64_t o = off[q].a;
+  for (int32_t r = 0; r < k; ++r, ++o) {
+    pk[o] = (ci << cbits) | (uec[s + r] & mask);
+    pv[o] = 0;
+  }
+}
+
+// Runs of equal (i, j) keys -> shared counts; the diagonal gives |N_i|.
+__global__ void run_heads(int64_t Z, const uint64_t* __restrict__ pk, int32_t* __restrict__ flag) {
+  const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (q >= Z) return;
+  flag[q] = q == 0 || pk[q - 1] != pk[q];
+}
+
+__global__ void run_table(int64_t Z, const uint64_t* __restrict__ pk,
+                          const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                          int64_t* __restrict__ head_at, uint64_t* __restrict__ ukey) {
+  const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (q < Z && flag[q]) {
+    head_at[pos[q]] = q;
+    ukey[pos[q]] = pk[q];
+  }
+}
+
+__global__ void run_counts(int32_t R, int64_t Z, const int64_t* __restrict__ head_at,
+                           const uint64_t* __restrict__ ukey, int cbits,
+                           int32_t* __restrict__ shared, int32_t* __restrict__ total,
+                           int32_t* __restrict__ offdiag, int32_t* __restrict__ row_cnt) {
+  const int32_t u = blockIdx.x * kB + threadIdx.x;
+  if (u >= R) return;
+  const int64_t end = u + 1 < R ? head_at[u + 1] : Z;
+  const int32_t c = (int32_t)(end - head_at[u]);
+  shared[u] = c;
+  const uint64_t mask = (1ull << cbits) - 1;
+  const int32_t i = (int32_t)(ukey[u] >> cbits), j = (int32_t)(ukey[u] & mask);
+  if (i == j) total[i] = c;
+  offdiag[u] = i != j;
+  if (i != j) atomicAdd(&row_cnt[i], 1);
+}
+
+// Off-diagonal entries in (i, j) order with their FP32 scores, and the
+// top-k sort keys (row, descending score).
+__global__ void scores_kernel(int32_t R, const uint64_t* __restrict__ ukey, int cbits,
+                              const int32_t* __restrict__ shared, const int32_t* __restrict__ total,
+                              const int32_t* __restrict__ offdiag, const int32_t* __restrict__ pos,
+                              int32_t* __restrict__ col, int32_t* __restrict__ shr,
+                              float* __restrict__ score, uint64_t* __restrict__ tkey,
+                              uint32_t* __restrict__ tval) {
+  const int32_t u = blockIdx.x * kB + threadIdx.x;
+  if (u >= R || !offdiag[u]) return;
+  const uint64_t mask = (1ull << cbits) - 1;
+  const int32_t i = (int32_t)(ukey[u] >> cbits), j = (int32_t)(ukey[u] & mask);
+  const int32_t o = pos[u];
+  const float s = (float)shared[u] / (float)total[i];
+  col[o] = j;
+  shr[o] = shared[u];
+  score[o] = s;
+  tkey[o] = ((uint64_t)i << 32) | (uint32_t)~order_bits(s);
+  tval[o] = (uint32_t)o;
+}
+
+__global__ void topk_kernel(int32_t Zo, const uint64_t* __restrict__ tkey,
+                            const uint32_t* __restrict__ tval, const int32_t* __restrict__ row_ptr,
+                            const int32_t* __restrict__ col, int32_t k, int32_t* __restrict__ top) {
+  const int32_t q = blockIdx.x * kB + threadIdx.x;
+  if (q >= Zo) return;
+  const int32_t i = (int32_t)(tkey[q] >> 32);
+  const int32_t rank = q - row_ptr[i];
+  if (rank < k) top[(int64_t)i * k + rank] = col[tval[q]];
+}
+
+int bits_for(int64_t n) {  // smallest b with n < 2^b, at least 1
+  int b = 1;
+  while (b < 62 && (1ll << b) <= n) ++b;
+  return b;
+}
+
+struct Bufs {
+  DevBuf b;
+  size_t cap = 0;
+  int get(fx_ctx* ctx, size_t bytes) {
+    if (bytes <= cap) return FX_OK;
+    b.release();
+    cap = 0;
+    int rc = b.alloc(ctx, std::max<size_t>(bytes, 256));
+    if (!rc) cap = std::max<size_t>(bytes, 256);
+    return rc;
+  }
+  template <typename T> T* as() const { return b.as<T>(); }
+};
+
+}  // namespace
+}  // namespace fx
+
+extern "C" int fx_attention_build(fx_ctx* ctx, const fx_netlist* net, int32_t num_nodes,
+                                  const int32_t* order, const int32_t* node_start,
+                                  const int32_t* node_count, const int32_t* node_depth,
+                                  int32_t top_k, fx_attention** out) {
+  using namespace fx;
+  clear_error();
+  FX_CHECK(ctx && net && out && num_nodes >= 0 && top_k >= 0 &&
+               (num_nodes == 0 || (node_start && node_count && node_depth)),
+           FX_E_INVALID, "fx_attention_build: bad arguments");
+  *out = nullptr;
+  const int32_t N = net->num_gates, E = net->num_nets;
+  FX_CHECK(N >= 0 && E >= 0 && (E == 0 || (net->net_ptr && net->net_pins)) && (N == 0 || order),
+           FX_E_INVALID, "fx_attention_build: bad netlist");
+  const int64_t P = E ? net->net_ptr[E] : 0;
+  FX_CHECK(P >= 0 && P < INT32_MAX, FX_E_INVALID, "fx_attention_build: bad net_ptr");
+  for (int64_t p = 0; p < P; ++p)
+    FX_CHECK(net->net_pins[p] >= 0 && net->net_pins[p] < N, FX_E_INVALID, "pin %lld out of range",
+             (long long)p);
+  int32_t max_depth = 0;
+  for (int32_t i = 0; i < num_nodes; ++i) {
+    FX_CHECK(node_start[i] >= 0 && node_count[i] >= 0 && node_start[i] + node_count[i] <= N &&
+                 node_depth[i] >= 0,
+             FX_E_INVALID, "node %d out of range", i);
+    max_depth = std::max(max_depth, node_depth[i]);
+  }
+  std::vector<std::vector<int32_t>> by_depth(max_depth + 1);
+  for (int32_t i = 0; i < num_nodes; ++i) by_depth[node_depth[i]].push_back(i);
+
+  auto* A = new fx_attention();
+  A->top_k = top_k;
+  std::unique_ptr<fx_attention> guard(A);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  FX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  DevBuf d_ptr, d_pins, d_order, d_pin_net, d_cl, d_ranges, d_kstart, d_kcnt;
+  Bufs keys, vals, flag, pos, uec, cnt, off, pk, pv, head_at, ukey, shared, total, offd, opos,
+      rowc, rowp, col, shr, score, tkey, tval, top, tmp;
+  SortWorkspace ws;
+  int rc;
+  if ((rc = d_ptr.alloc(ctx, 4 * ((size_t)E + 1))) || (rc = d_pins.alloc(ctx, 4 * (size_t)P + 4)) ||
+      (rc = d_order.alloc(ctx, 4 * (size_t)N + 4)) || (rc = d_pin_net.alloc(ctx, 4 * (size_t)P + 4)) ||
+      (rc = d_cl.alloc(ctx, 4 * (size_t)N + 4)) || (rc = d_kstart.alloc(ctx, 4 * (size_t)E + 4)) ||
+      (rc = d_kcnt.alloc(ctx, 4 * (size_t)E + 4)))
+    return rc;
+  if (E) FX_HIP(hipMemcpyAsync(d_ptr.p, net->net_ptr, 4 * ((size_t)E + 1), hipMemcpyHostToDevice, st));
+  if (P) FX_HIP(hipMemcpyAsync(d_pins.p, net->net_pins, 4 * (size_t)P, hipMemcpyHostToDevice, st));
+  if (N) FX_HIP(hipMemcpyAsync(d_order.p, order, 4 * (size_t)N, hipMemcpyHostToDevice, st));
+  if (E)
+    hipLaunchKernelGGL(pin_net_kernel, dim3(div_up(E, kB)), dim3(kB), 0, st, E, d_ptr.as<int32_t>(),
+                       d_pin_net.as<int32_t>());
+  FX_HIP(hipEventRecord(ctx->ev0, st));
+  const dim3 blk(kB);
+  for (int32_t d = 0; d <= max_depth; ++d) {
+    const std::vector<int32_t>& ids = by_depth[d];
+    const int32_t n = (int32_t)ids.size();
+    if (n < 2) continue;  // attention.rs:40-43
+    A->levels.emplace_back();
+    fx_attention::Level& L = A->levels.back();
+    L.depth = d;
+    L.nodes = ids;
+    L.total.assign(n, 0);
+    L.row_ptr.assign((size_t)n + 1, 0);
+    L.top.assign((size_t)n * top_k, -1);
+    if (P == 0) continue;
+    const int cbits = bits_for(n);
+    const int ebits = bits_for(E);
+    FX_CHECK(cbits + ebits <= 62 && 2 * cbits <= 62, FX_E_INVALID, "attention: too many clusters");
+    // 1. cluster of every gate at this depth
+    std::vector<int2> ranges(n);
+    for (int32_t c = 0; c < n; ++c) ranges[c] = make_int2(node_start[ids[c]], node_count[ids[c]]);
+    DevBuf d_rng;
+    if ((rc = d_rng.alloc(ctx, sizeof(int2) * n))) return rc;
+    FX_HIP(hipMemcpyAsync(d_rng.p, ranges.data(), sizeof(int2) * n, hipMemcpyHostToDevice, st));
+    if ((rc = fill_i32(st, d_cl.as<int32_t>(), N, -1))) return rc;
+    hipLaunchKernelGGL(label_kernel, dim3(n), blk, 0, st, d_rng.as<int2>(), d_order.as<int32_t>(),
+                       d_cl.as<int32_t>());
+    // 2. distinct (net, cluster) pairs
+    if ((rc = keys.get(ctx, 8 * (size_t)P)) || (rc = vals.get(ctx, 4 * (size_t)P)) ||
+        (rc = flag.get(ctx, 4 * (size_t)P)) || (rc = pos.get(ctx, 4 * ((size_t)P + 1))) ||
+        (rc = uec.get(ctx, 8 * (size_t)P)) || (rc = tmp.get(ctx, 16 * (size_t)scan_tmp_elems(P))))
+      return rc;
+    hipLaunchKernelGGL(pin_keys_kernel, dim3(div_up(P, kB)), blk, 0, st, P, d_pins.as<int32_t>(),
+                       d_pin_net.as<int32_t>(), d_cl.as<int32_t>(), cbits, keys.as<uint64_t>(),
+                       vals.as<uint32_t>());
+    if ((rc = radix_sort_pairs(ctx, st, ws, keys.as<uint64_t>(), vals.as<uint32_t>(), P, 0,
+                               ebits + cbits)))
+      return rc;
+    hipLaunchKernelGGL(head_flags, dim3(div_up(P, kB)), blk, 0, st, P, keys.as<uint64_t>(),
+                       flag.as<int32_t>());
+    if ((rc = scan_exclusive_i32(st, flag.as<int32_t>(), pos.as<int32_t>(), P, tmp.as<int32_t>(),
+                                 scan_tmp_elems(P))))
+      return rc;
+    hipLaunchKernelGGL(compact_keys, dim3(div_up(P, kB)), blk, 0, st, P, keys.as<uint64_t>(),
+                       flag.as<int32_t>(), pos.as<int32_t>(), uec.as<uint64_t>());
+    int32_t U = 0;
+    FX_HIP(hipMemcpyAsync(&U, pos.as<int32_t>() + P, 4, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    if (U == 0) continue;
+    // 3. every net's ordered cluster pairs
+    if ((rc = fill_i32(st, d_kcnt.as<int32_t>(), E, 0))) return rc;
+    hipLaunchKernelGGL(net_runs, dim3(div_up(U, kB)), blk, 0, st, (int64_t)U, uec.as<uint64_t>(), cbits,
+                       d_kstart.as<int32_t>(), d_kcnt.as<int32_t>());
+    if ((rc = cnt.get(ctx, 16 * (size_t)U)) || (rc = off.get(ctx, 16 * ((size_t)U + 1))) ||
+        (rc = tmp.get(ctx, 16 * (size_t)scan_tmp_elems(U))))
+      return rc;
+    hipLaunchKernelGGL(pair_counts, dim3(div_up(U, kB)), blk, 0, st, (int64_t)U, uec.as<uint64_t>(),
+                       cbits, d_kcnt.as<int32_t>(), cnt.as<I64x2d>());
+    if ((rc = scan_exclusive_i64x2(st, cnt.as<void>(), off.as<void>(), U, tmp.as<void>(),
+                                   scan_tmp_elems(U))))
+      return rc;
+    I64x2d zt{0, 0};
+    FX_HIP(hipMemcpyAsync(&zt, off.as<I64x2d>() + U, 16, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    const int64_t Z = zt.a;
+    FX_CHECK(Z > 0 && Z < (int64_t)UINT32_MAX, FX_E_NOMEM,
+             "attention: %lld cluster pairs at depth %d", (long long)Z, d);
+    if ((rc = pk.get(ctx, 8 * (size_t)Z)) || (rc = pv.get(ctx, 4 * (size_t)Z)))
+      return rc;
+    hipLaunchKernelGGL(emit_pairs, dim3(div_up(U, kB)), blk, 0, st, (int64_t)U, uec.as<uint64_t>(),
+                       cbits, d_kstart.as<int32_t>(), d_kcnt.as<int32_t>(), off.as<I64x2d>(),
+                       pk.as<uint64_t>(), pv.as<uint32_t>());
+    if ((rc = radix_sort_pairs(ctx, st, ws, pk.as<uint64_t>(), pv.as<uint32_t>(), Z, 0, 2 * cbits)))
+      return rc;
+    // 4. runs -> shared counts, |N_i|, off-diagonal CSR
+    if ((rc = flag.get(ctx, 4 * (size_t)Z)) || (rc = pos.get(ctx, 4 * ((size_t)Z + 1))) ||
+        (rc = tmp.get(ctx, 16 * (size_t)scan_tmp_elems(Z))))
+      return rc;
+    hipLaunchKernelGGL(run_heads, dim3(div_up(Z, kB)), blk, 0, st, Z, pk.as<uint64_t>(),
+                       flag.as<int32_t>());
+    if ((rc = scan_exclusive_i32(st, flag.as<int32_t>(), pos.as<int32_t>(), Z, tmp.as<int32_t>(),
+                                 scan_tmp_elems(Z))))
+      return rc;
+    int32_t R = 0;
+    FX_HIP(hipMemcpyAsync(&R, pos.as<int32_t>() + Z, 4, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    if ((rc = head_at.get(ctx, 8 * (size_t)R)) || (rc = ukey.get(ctx, 8 * (size_t)R)) ||
+        (rc = shared.get(ctx, 4 * (size_t)R)) || (rc = total.get(ctx, 4 * (size_t)n)) ||
+        (rc = offd.get(ctx, 4 * (size_t)R)) || (rc = opos.get(ctx, 4 * ((size_t)R + 1))) ||
+        (rc = rowc.get(ctx, 4 * (size_t)n)) || (rc = rowp.get(ctx, 4 * ((size_t)n + 1))) ||
+        (rc = tmp.get(ctx, 16 * (size_t)scan_tmp_elems(std::max<int64_t>(R, n)))))
+      return rc;
+    hipLaunchKernelGGL(run_table, dim3(div_up(Z, kB)), blk, 0, st, Z, pk.as<uint64_t>(),
+                       flag.as<int32_t>(), pos.as<int32_t>(), head_at.as<int64_t>(),
+                       ukey.as<uint64_t>());
+    if ((rc = fill_i32(st, total.as<int32_t>(), n, 0)) || (rc = fill_i32(st, rowc.as<int32_t>(), n, 0)))
+      return rc;
+    hipLaunchKernelGGL(run_counts, dim3(div_up(R, kB)), blk, 0, st, R, Z, head_at.as<int64_t>(),
+                       ukey.as<uint64_t>(), cbits, shared.as<int32_t>(), total.as<int32_t>(),
+                       offd.as<int32_t>(), rowc.as<int32_t>());
+    if ((rc = scan_exclusive_i32(st, offd.as<int32_t>(), opos.as<int32_t>(), R, tmp.as<int32_t>(),
+                                 scan_tmp_elems(R))) ||
+        (rc = scan_exclusive_i32(st, rowc.as<int32_t>(), rowp.as<int32_t>(), n, tmp.as<int32_t>(),
+                                 scan_tmp_elems(n))))
+      return rc;
+    int32_t Zo = 0;
+    FX_HIP(hipMemcpyAsync(&Zo, opos.as<int32_t>() + R, 4, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    if ((rc = col.get(ctx, 4 * (size_t)Zo)) || (rc = shr.get(ctx, 4 * (size_t)Zo)) ||
+        (rc = score.get(ctx, 4 * (size_t)Zo)) || (rc = tkey.get(ctx, 8 * (size_t)Zo)) ||
+        (rc = tval.get(ctx, 4 * (size_t)Zo)) || (rc = top.get(ctx, 4 * (size_t)n * std::max(top_k, 1))))
+      return rc;
+    hipLaunchKernelGGL(scores_kernel, dim3(div_up(R, kB)), blk, 0, st, R, ukey.as<uint64_t>(), cbits,
+                       shared.as<int32_t>(), total.as<int32_t>(), offd.as<int32_t>(),
+                       opos.as<int32_t>(), col.as<int32_t>(), shr.as<int32_t>(), score.as<float>(),
+                       tkey.as<uint64_t>(), tval.as<uint32_t>());
+    // 5. top-k: stable sort by (row, descending score) keeps ascending j on ties
+    if (top_k > 0 && Zo > 0) {
+      if ((rc = radix_sort_pairs(ctx, st, ws, tkey.as<uint64_t>(), tval.as<uint32_t>(), Zo, 0,
+                                 32 + cbits)))
+        return rc;
+      if ((rc = fill_i32(st, top.as<int32_t>(), (int64_t)n * top_k, -1))) return rc;
+      hipLaunchKernelGGL(topk_kernel, dim3(div_up(Zo, kB)), blk, 0, st, Zo, tkey.as<uint64_t>(),
+                         tval.as<uint32_t>(), rowp.as<int32_t>(), col.as<int32_t>(), top_k,
+                         top.as<int32_t>());
+    }
+    FX_HIP(hipGetLastError());
+    std::vector<int32_t> rp32((size_t)n + 1);
+    L.col.resize(Zo);
+    L.shared.resize(Zo);
+    L.score.resize(Zo);
+    FX_HIP(hipMemcpyAsync(L.total.data(), total.as<int32_t>(), 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipMemcpyAsync(rp32.data(), rowp.as<int32_t>(), 4 * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
+    if (Zo) {
+      FX_HIP(hipMemcpyAsync(L.col.data(), col.as<int32_t>(), 4 * (size_t)Zo, hipMemcpyDeviceToHost, st));
+      FX_HIP(hipMemcpyAsync(L.shared.data(), shr.as<int32_t>(), 4 * (size_t)Zo, hipMemcpyDeviceToHost, st));
+      FX_HIP(hipMemcpyAsync(L.score.data(), score.as<float>(), 4 * (size_t)Zo, hipMemcpyDeviceToHost, st));
+    }
+    if (top_k > 0 && Zo > 0)
+      FX_HIP(hipMemcpyAsync(L.top.data(), top.as<int32_t>(), 4 * (size_t)n * top_k,
+                            hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    for (int32_t i = 0; i <= n; ++i) L.row_ptr[i] = rp32[i];
+  }
+  FX_HIP(hipEventRecord(ctx->ev1, st));
+  FX_HIP(hipStreamSynchronize(st));
+  bool faulted = false;
+  if ((rc = radix_fault(ws, st, &faulted))) return rc;
+  FX_CHECK(!faulted, FX_E_DEVICE, "radix sort look-back timed out");
+  float ms = 0.f;
+  FX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->attention_ms = ms;
+  *out = guard.release();
+  return FX_OK;
+}
+
+extern "C" void fx_attention_free(fx_attention* a) { delete a; }
+
+extern "C" int32_t fx_attention_num_levels(const fx_attention* a) {
+  return a ? (int32_t)a->levels.size() : 0;
+}
+
+extern "C" int fx_attention_level_info(const fx_attention* a, int32_t level, int32_t* level_index,
+                                       int32_t* num_clusters, int64_t* nnz) {
+  fx::clear_error();
+  FX_CHECK(a && level >= 0 && level < (int32_t)a->levels.size(), FX_E_INVALID,
+           "fx_attention_level_info: bad level %d", level);
+  const auto& L = a->levels[level];
+  if (level_index) *level_index = L.depth;
+  if (num_clusters) *num_clusters = (int32_t)L.nodes.size();
+  if (nnz) *nnz = (int64_t)L.col.size();
+  return FX_OK;
+}
+
+extern "C" int fx_attention_level_get(const fx_attention* a, int32_t level, int32_t* cluster_node,
+                                      int32_t* total, int64_t* row_ptr, int32_t* col,
+                                      int32_t* shared, float* score, int32_t* top) {
+  fx::clear_error();
+  FX_CHECK(a && level >= 0 && level < (int32_t)a->levels.size(), FX_E_INVALID,
+           "fx_attention_level_get: bad level %d", level);
+  const auto& L = a->levels[level];
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst && !v.empty()) std::copy(v.begin(), v.end(), dst);
+  };
+  cp(cluster_node, L.nodes);
+  cp(total, L.total);
+  cp(row_ptr, L.row_ptr);
+  cp(col, L.col);
+  cp(shared, L.shared);
+  cp(score, L.score);
+  cp(top, L.top);
+  return FX_OK;
+}
+
+extern "C" double fx_attention_last_ms(const fx_ctx* ctx) { return ctx ? ctx->attention_ms : 0.0; }

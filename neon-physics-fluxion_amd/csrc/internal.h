@@ -23,6 +23,7 @@ struct fx_ctx {
   fx_leaf_tile leaf_tile[2] = {{1, 64, 256}, {1, 64, 256}};  // per leaf mode
   double fiedler_ms = 0.0;
   double hierarchy_ms = 0.0;
+  double attention_ms = 0.0;
   std::mutex mu;  // serialises calls that use the context's own stream
 };
 

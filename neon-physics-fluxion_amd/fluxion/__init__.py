@@ -7,6 +7,7 @@ from .placer import (Circuit, FlatJob, GpuContext, Placement, PlacementServer,  
                      refine_placement, set_leaves_tile, sort_f32, sort_u64, split_by_fiedler)
 from .hierarchy import (ClusterNode, HierarchyTree, PartitionConfig, build_hierarchy,  # noqa: F401
                         hierarchy_size)
+from .attention import AttentionConfig, AttentionLevel, AttentionMap, compute_all_attention  # noqa: F401
 from .placer import hapr_placement  # noqa: F401
 from .parser import parse_blif, write_def  # noqa: F401
 from .engine import FluxionEngine  # noqa: F401

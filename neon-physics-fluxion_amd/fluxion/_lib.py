@@ -172,6 +172,14 @@ _SIGS = {
                                      _P, _P, _P, _P, _I32, C.POINTER(_I32)]),
     "fx_hierarchy_last_ms": (_D, [_P]),
     "fx_hierarchy_regions": (C.c_int, [_I32, _P, _P, _F, _F, _P]),
+    "fx_attention_build": (C.c_int, [_P, C.POINTER(FxNetlist), _I32, _P, _P, _P, _P, _I32,
+                                     C.POINTER(_P)]),
+    "fx_attention_num_levels": (_I32, [_P]),
+    "fx_attention_level_info": (C.c_int, [_P, _I32, C.POINTER(_I32), C.POINTER(_I32),
+                                          C.POINTER(_I64)]),
+    "fx_attention_level_get": (C.c_int, [_P, _I32, _P, _P, _P, _P, _P, _P, _P]),
+    "fx_attention_free": (None, [_P]),
+    "fx_attention_last_ms": (_D, [_P]),
     "fx_blif_read": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "fx_circuit_free": (None, [_P]),
     "fx_circuit_info": (C.c_int, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64),

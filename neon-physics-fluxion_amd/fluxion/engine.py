@@ -4,9 +4,10 @@ The reference's engine shells out to the Rust CLI, which runs main.rs:27-121:
 parse -> build_hierarchy -> attention -> hapr_placement -> refine_placement ->
 write_def. Here the same pipeline runs in-process on the GPU library:
 BLIF and DEF in the library's host code, the bisection, the leaf placement
-and the refinement on the GPU. Attention scoring (main.rs:88-93) is skipped:
-its only consumer is the depth-0 spectral branch of hapr_placement, which
-never runs (the root is the only cluster of depth 0, attention.rs:40-43).
+and the refinement on the GPU. Attention scoring (main.rs:88-93) runs only
+when asked for (attention=True): its only consumer is the depth-0 spectral
+branch of hapr_placement, which never runs (the root is the only cluster of
+depth 0, attention.rs:40-43), so the placement is the same either way.
 """
 from __future__ import annotations
 
@@ -15,6 +16,7 @@ from pathlib import Path
 
 import yaml
 
+from .attention import AttentionConfig, compute_all_attention
 from .hierarchy import PartitionConfig, build_hierarchy
 from .parser import parse_blif, write_def
 from .placer import GpuContext, PlacerConfig, RefineConfig, hapr_placement, refine_placement
@@ -37,7 +39,7 @@ class FluxionEngine:
     """
 
     def __init__(self, config_path="default.yaml", ctx: GpuContext | None = None,
-                 exact: bool = True):
+                 exact: bool = True, attention: bool = False):
         self.config_path = Path(config_path)
         if not self.config_path.exists():
             raise FileNotFoundError(f"Config not found: {self.config_path}")
@@ -45,6 +47,8 @@ class FluxionEngine:
             self.config = yaml.safe_load(f) or {}
         self._ctx = ctx
         self.exact = exact
+        self.attention = attention  # main.rs:88-93; no effect on the placement (Q3)
+        self.attention_map = None
         self.stats: dict = {}
 
     # Config keys of config/default.yaml (config.rs:7-72); missing keys take
@@ -90,6 +94,11 @@ class FluxionEngine:
         t1 = time.perf_counter()
         ctx = self._context()
         tree = build_hierarchy(ctx, circuit, self.partition_config(), exact=self.exact)
+        ta = time.perf_counter()
+        if self.attention:
+            a = _section(self.config, "attention")
+            self.attention_map = compute_all_attention(
+                ctx, tree, circuit, AttentionConfig(int(a.get("top_k", 10))))
         t2 = time.perf_counter()
         placement = hapr_placement(ctx, circuit, tree, None, self.placer_config(),
                                    fast=not self.exact)
@@ -99,7 +108,7 @@ class FluxionEngine:
         t4 = time.perf_counter()
         self.stats = {"gates": circuit.num_gates, "nets": circuit.num_nets,
                       "leaves": int((tree.node_left < 0).sum()), "max_depth": tree.max_depth,
-                      "parse_s": t1 - t0, "hierarchy_s": t2 - t1,
+                      "parse_s": t1 - t0, "hierarchy_s": ta - t1, "attention_s": t2 - ta,
                       "hierarchy_gpu_ms": tree.build_ms, "placement_s": t3 - t2,
                       "refine_s": t4 - t3, "hpwl": placement.total_wirelength,
                       "refine": summary}

@@ -1353,3 +1353,59 @@ ORC_API void orc_hapr_regions(int32_t num_nodes, const int32_t* depth, const int
   RegionTree t{depth, left, right, region};
   for (int32_t d = 1; d <= max_depth; ++d) t.place_level(0, d);
 }
+
+// ---------------------------------------------------------------------------
+// compute_level_attention (HAPR_fluxion_v4/src/algorithm/attention.rs:50-88)
+// for one level's clusters (cluster c = order[cl_start[c] .. +cl_count[c]]),
+// literally: the set of nets touching each cluster (nets_touching_gates,
+// 101-110: every net with any gate in the cluster), shared = the size of
+// the intersection of two such sets, score = shared as f32 / total as f32,
+// rows of clusters touching no net left at zero; top_k by a stable sort on
+// descending score over j != i, the first k kept, zero scores dropped.
+// scores: [n*n] row-major; top: [n*k], -1 padded.
+// ---------------------------------------------------------------------------
+ORC_API void orc_attention_level(int32_t num_gates, int32_t num_nets, const int32_t* net_ptr,
+                                 const int32_t* net_pins, int32_t n, const int32_t* cl_start,
+                                 const int32_t* cl_count, const int32_t* order, int32_t top_k,
+                                 float* scores, int32_t* top) {
+  std::vector<std::vector<int32_t>> nets(n);  // sorted net ids per cluster
+  std::vector<char> in_cluster(num_gates, 0);
+  for (int32_t c = 0; c < n; ++c) {
+    for (int32_t i = 0; i < cl_count[c]; ++i) in_cluster[order[cl_start[c] + i]] = 1;
+    for (int32_t e = 0; e < num_nets; ++e)
+      for (int32_t p = net_ptr[e]; p < net_ptr[e + 1]; ++p)
+        if (in_cluster[net_pins[p]]) {
+          nets[c].push_back(e);
+          break;
+        }
+    for (int32_t i = 0; i < cl_count[c]; ++i) in_cluster[order[cl_start[c] + i]] = 0;
+  }
+  for (int64_t k = 0; k < (int64_t)n * n; ++k) scores[k] = 0.0f;
+  for (int32_t i = 0; i < n; ++i) {
+    const size_t total = nets[i].size();
+    if (total == 0) continue;
+    for (int32_t j = 0; j < n; ++j) {
+      if (i == j) continue;
+      size_t shared = 0, a = 0, b = 0;
+      const auto& A = nets[i];
+      const auto& B = nets[j];
+      while (a < A.size() && b < B.size()) {
+        if (A[a] < B[b]) ++a;
+        else if (B[b] < A[a]) ++b;
+        else { ++shared; ++a; ++b; }
+      }
+      scores[(int64_t)i * n + j] = (float)shared / (float)total;
+    }
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    std::vector<int32_t> idx;
+    for (int32_t j = 0; j < n; ++j)
+      if (j != i) idx.push_back(j);
+    const float* row = scores + (int64_t)i * n;
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return row[b] < row[a]; });
+    int32_t t = 0;
+    for (int32_t r = 0; r < (int32_t)idx.size() && r < top_k; ++r)
+      if (row[idx[r]] > 0.0f) top[(int64_t)i * top_k + t++] = idx[r];
+    for (; t < top_k; ++t) top[(int64_t)i * top_k + t] = -1;
+  }
+}

@@ -43,6 +43,7 @@ _SIGS = {
                                    _P, _I32]),
     "orc_fiedler_vector": (None, [_I32, _P, _I32, _P, _P, _I32, _I32, _P]),
     "orc_hapr_regions": (None, [_I32, _P, _P, _P, _F, _F, _P]),
+    "orc_attention_level": (None, [_I32, _I32, _P, _P, _I32, _P, _P, _P, _I32, _P, _P]),
 }
 
 
@@ -290,3 +291,16 @@ def hapr_regions(nodes, chip_w, chip_h):
     lib.orc_hapr_regions(nn, p(i32(nodes["depth"])), p(i32(nodes["left"])), p(i32(nodes["right"])),
                          chip_w, chip_h, p(out))
     return out[:nn]
+
+
+def attention_level(num_gates, net_ptr, net_pins, cl_start, cl_count, order, top_k=10):
+    """compute_level_attention (attention.rs:50-88) for one level's clusters:
+    (dense scores [n, n] f32, top [n, k] with -1 padding)."""
+    n = len(cl_start)
+    scores = np.zeros((n, n), np.float32)
+    top = np.zeros((n, max(top_k, 1)), np.int32)
+    pins = i32(net_pins) if len(net_pins) else np.zeros(1, np.int32)
+    lib.orc_attention_level(num_gates, len(net_ptr) - 1, p(i32(net_ptr)), p(pins), n,
+                            p(i32(cl_start)), p(i32(cl_count)), p(i32(order)), top_k, p(scores),
+                            p(top))
+    return scores, top[:, :top_k]

@@ -308,3 +308,49 @@ def test_engine_config(tmp_path):
         fxn.FluxionEngine(tmp_path / "nope.yaml")
     with pytest.raises(FileNotFoundError):
         eng.place(tmp_path / "nope.blif", tmp_path / "o.def")
+
+
+# ---- oracle: attention (attention.rs:50-88) ----
+
+def test_oracle_attention_reference_case():
+    # attention.rs:122-143 (test_attention_basic): 6 gates, nets n0 = [0, 1],
+    # n1 = [0, 3], n2 = [3, 4]; clusters {0, 1, 2} and {3, 4, 5}. Each cluster
+    # touches two nets and shares n1: scores 1/2 both ways.
+    c = fxn.Circuit.from_nets(6, [[0, 1], [0, 3], [3, 4]], 100.0, 100.0)
+    scores, top = orc.attention_level(6, c.net_ptr, c.net_pins, [0, 3], [3, 3], np.arange(6), 5)
+    assert np.array_equal(scores, np.array([[0, 0.5], [0.5, 0]], np.float32))
+    assert top.tolist() == [[1, -1, -1, -1, -1], [0, -1, -1, -1, -1]]
+
+
+def test_oracle_attention_kat():
+    # A = {0,1,2}, B = {3,4,5}, C = {6,7,8}, D = {9}; nets e0 = [0,3] (A,B),
+    # e1 = [1,6] (A,C), e2 = [2] (single pin, A), e3 = [4,5] (B), e4 = [7,8] (C).
+    # N_A = {e0,e1,e2}, N_B = {e0,e3}, N_C = {e1,e4}, N_D = {}.
+    nets = [[0, 3], [1, 6], [2], [4, 5], [7, 8]]
+    c = fxn.Circuit.from_nets(10, nets)
+    order = np.arange(10)
+    scores, top = orc.attention_level(10, c.net_ptr, c.net_pins, [0, 3, 6, 9], [3, 3, 3, 1], order, 3)
+    third = np.float32(1) / np.float32(3)
+    want = np.array([[0, third, third, 0], [0.5, 0, 0, 0], [0.5, 0, 0, 0], [0, 0, 0, 0]], np.float32)
+    assert np.array_equal(scores, want)
+    # ties keep ascending j; zero scores dropped; D has no nets: empty row
+    assert top.tolist() == [[1, 2, -1], [0, -1, -1], [0, -1, -1], [-1, -1, -1]]
+    _, top1 = orc.attention_level(10, c.net_ptr, c.net_pins, [0, 3, 6, 9], [3, 3, 3, 1], order, 1)
+    assert top1.tolist() == [[1], [0], [0], [-1]]
+
+
+def test_oracle_attention_symmetry():
+    # shared counts are symmetric: score_ij * total_i == score_ji * total_j
+    circ, _, _ = fxn.netgen.flat(600, 660, 4)
+    starts = np.arange(0, 600, 50)
+    counts = np.full(12, 50)
+    scores, _ = orc.attention_level(600, circ.net_ptr, circ.net_pins, starts, counts,
+                                    np.arange(600), 10)
+    totals = []
+    for s in starts:
+        g = set(range(s, s + 50))
+        totals.append(sum(1 for e in range(circ.num_nets)
+                          if g & set(circ.net_pins[circ.net_ptr[e]:circ.net_ptr[e + 1]].tolist())))
+    shared = scores.astype(np.float64) * np.array(totals)[:, None]
+    assert np.allclose(shared, np.rint(shared), atol=1e-3)
+    assert np.array_equal(np.rint(shared), np.rint(shared).T)
