@@ -136,6 +136,31 @@ def test_barnes_hut_colocated_and_clamped():
     print(f"record vs literal f64: {max(nrm_err(rx, lx), nrm_err(ry, ly)):.2e}", flush=True)
 
 
+def test_barnes_hut_clustered():
+    # Dense clusters: the key sort's buckets (cells of the first 8 tree levels)
+    # then hold tens of thousands of keys (merge path), hundreds (LDS path)
+    # and a few (wave path); the walk stays bit-identical to the path of record.
+    print("case: BH clustered", flush=True)
+    rng = np.random.default_rng(12)
+    N = 80000
+    x = rng.uniform(0, 1000, N).astype(np.float32)
+    y = rng.uniform(0, 1000, N).astype(np.float32)
+    x[:50000] = rng.normal(400.0, 0.4, 50000).astype(np.float32)
+    y[:50000] = rng.normal(600.0, 0.4, 50000).astype(np.float32)
+    x[50000:56000] = rng.normal(100.0, 6.0, 6000).astype(np.float32)
+    y[50000:56000] = rng.normal(900.0, 6.0, 6000).astype(np.float32)
+    x[56000:56100] = 250.0
+    y[56000:56100] = 250.0
+    circ = fxn.Circuit.from_nets(N, [[i, (i + 1) % N] for i in range(N)])
+    job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig())
+    job.set_positions(x, y)
+    gx, gy = job.forces(2)
+    rx, ry, st = orc.bh_record(x, y, nthreads=8)
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    s = job.bh_stats()
+    assert s["nodes"] == st["nodes"] and s["interactions"] == st["interactions"]
+
+
 @pytest.mark.parametrize("G", [32, 64, 128, 512])
 def test_density_fields_vs_fp64(G):
     print(f"case: DCT G={G}", flush=True)
