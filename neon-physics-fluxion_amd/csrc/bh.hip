@@ -85,30 +85,45 @@ __global__ __launch_bounds__(kBlock) void bh_bbox_partial(int n, const float* __
     else fb[i - na] = vb;
   }
   float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  float nz = INFINITY, bad = 0.0f;  // smallest non-zero magnitude; 1 if a NaN was seen
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    mnx = fminf(mnx, x[i]);
-    mxx = fmaxf(mxx, x[i]);
-    mny = fminf(mny, y[i]);
-    mxy = fmaxf(mxy, y[i]);
+    const float xi = x[i], yi = y[i];
+    mnx = fminf(mnx, xi);
+    mxx = fmaxf(mxx, xi);
+    mny = fminf(mny, yi);
+    mxy = fmaxf(mxy, yi);
+    if (xi != 0.0f) nz = fminf(nz, fabsf(xi));
+    if (yi != 0.0f) nz = fminf(nz, fabsf(yi));
+    if (xi != xi || yi != yi) bad = 1.0f;
   }
   for (int off = 32; off > 0; off >>= 1) {
     mnx = fminf(mnx, __shfl_down(mnx, off, 64));
     mxx = fmaxf(mxx, __shfl_down(mxx, off, 64));
     mny = fminf(mny, __shfl_down(mny, off, 64));
     mxy = fmaxf(mxy, __shfl_down(mxy, off, 64));
+    nz = fminf(nz, __shfl_down(nz, off, 64));
+    bad = fmaxf(bad, __shfl_down(bad, off, 64));
   }
   __shared__ float4 sh[kBlock / 64];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = make_float4(mnx, mxx, mny, mxy);
+  __shared__ float2 sa[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = make_float4(mnx, mxx, mny, mxy);
+    sa[threadIdx.x >> 6] = make_float2(nz, bad);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     float4 r = sh[0];
+    float2 a = sa[0];
     for (int w = 1; w < kBlock / 64; ++w) {
       r.x = fminf(r.x, sh[w].x);
       r.y = fmaxf(r.y, sh[w].y);
       r.z = fminf(r.z, sh[w].z);
       r.w = fmaxf(r.w, sh[w].w);
+      a.x = fminf(a.x, sa[w].x);
+      a.y = fmaxf(a.y, sa[w].y);
     }
     part[blockIdx.x] = r;
+    part[gridDim.x + blockIdx.x] = make_float4(a.x, a.y, 0.0f, 0.0f);
   }
 }
 
@@ -122,19 +137,27 @@ __device__ void geometry_from_boxes(const float4* __restrict__ part, int nparts,
                                     BhGeometry& g) {
   const int lane = threadIdx.x & 63;
   float4 r = part[lane < nparts ? lane : 0];
+  float4 a = part[nparts + (lane < nparts ? lane : 0)];
   for (int k = lane + 64; k < nparts; k += 64) {
     r.x = fminf(r.x, part[k].x);
     r.y = fmaxf(r.y, part[k].y);
     r.z = fminf(r.z, part[k].z);
     r.w = fmaxf(r.w, part[k].w);
+    a.x = fminf(a.x, part[nparts + k].x);
+    a.y = fmaxf(a.y, part[nparts + k].y);
   }
   for (int off = 32; off > 0; off >>= 1) {
     r.x = fminf(r.x, __shfl_down(r.x, off, 64));
     r.y = fmaxf(r.y, __shfl_down(r.y, off, 64));
     r.z = fminf(r.z, __shfl_down(r.z, off, 64));
     r.w = fmaxf(r.w, __shfl_down(r.w, off, 64));
+    a.x = fminf(a.x, __shfl_down(a.x, off, 64));
+    a.y = fmaxf(a.y, __shfl_down(a.y, off, 64));
   }
   if (lane != 0) return;
+  // Coordinate range of the walk's shortened exact arithmetic (bh_node).
+  const float big = fmaxf(fmaxf(fabsf(r.x), fabsf(r.y)), fmaxf(fabsf(r.z), fabsf(r.w)));
+  g.safe_range = a.y == 0.0f && big <= 0x1p20f && a.x >= 0x1p-20f;
   const double min_x = (double)r.x - 1.0, max_x = (double)r.y + 1.0;
   const double min_y = (double)r.z - 1.0, max_y = (double)r.w + 1.0;
   const double size = fmax(max_x - min_x, max_y - min_y);
@@ -188,6 +211,7 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
         gout->scale_bits = gl.scale_bits;
         gout->scale = gl.scale;
         gout->size_f = gl.size_f;
+        gout->safe_range = gl.safe_range;
         gout->num_nodes = 0;  // overflow stays set for the rest of the run
       }
     }
@@ -472,14 +496,57 @@ struct WalkParams {
   // theta_m only for subnormal theta).
   double theta_m;
   int theta_tie_le;
+  // min_dist in [2^-20, 2^20] and k of magnitude in [2^-40, 2^10]:
+  // with coordinates in the walk's safe range (BhGeometry::safe_range) every
+  // square root and division of the exact walk then has operands for which
+  // the short sequences below are the IEEE results.
+  int short_ok;
 };
+
+// IEEE arithmetic on restricted operands, without the scaling and
+// special-value steps of the general sequences:
+//  * sqrt of x in [2^-96, 2^96]: the hardware estimate corrected by the
+//    residuals of its two neighbours (fma(-s', s, x) decides between s-1ulp,
+//    s and s+1ulp) — the general sequence for such x, which only adds a
+//    2^32 prescale for x < 2^-96 and returns 0, inf and NaN unchanged;
+//  * num/den for den of magnitude in [2^-40, 2^44] and num zero or of
+//    magnitude in [2^-100, 2^40]: the reciprocal estimate refined by one
+//    Newton step, then the quotient corrected twice by its exact residual
+//    fma(-den, q, num) — the general sequence without the operand scaling
+//    (never triggered for such operands) and the final special-case fixup
+//    (a no-op for normal quotients); a zero numerator gives itself.
+// The reciprocal depends on the divisor only, so two divisions by the same
+// divisor share it.
+__device__ __forceinline__ float sqrt_short(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __int_as_float(__float_as_int(s) - 1);
+  const float up = __int_as_float(__float_as_int(s) + 1);
+  const float s1 = fmaf(-dn, s, x) <= 0.0f ? dn : s;
+  return fmaf(-up, s, x) > 0.0f ? up : s1;
+}
+
+__device__ __forceinline__ float recip_short(float den) {
+  const float r = __builtin_amdgcn_rcpf(den);
+  const float e = fmaf(-den, r, 1.0f);
+  return fmaf(e, r, r);
+}
+
+template <bool kMaybeZero = true>
+__device__ __forceinline__ float div_short(float num, float den, float r) {
+  float q = num * r;
+  float e = fmaf(-den, q, num);
+  q = fmaf(e, r, q);
+  e = fmaf(-den, q, num);
+  q = fmaf(e, r, q);
+  return (kMaybeZero && num == 0.0f) ? num : q;
+}
 
 // Per-node interaction of the walk. Exact: the reference's operations with
 // IEEE square root and division (bit-identical to the FP32 path of record;
 // the opening test is the same predicate evaluated exactly in FP64).
 // Fast: the same law with v_rsq_f32 and FMA and the opening test squared
 // (size^2 < theta^2 max(d^2, min_dist^2)); forces within 1e-6 relative.
-template <bool kFast>
+template <bool kFast, bool kShort = false>
 __device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, float py,
                                         const WalkParams& wp, float size_root, float& ax,
                                         float& ay) {
@@ -496,7 +563,9 @@ __device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, f
     return true;
   }
   const float dist_sq = dx * dx + dy * dy;
-  float dist = sqrtf(dist_sq);
+  // Short form: below 2^-96 the square root is under 2^-48 and is replaced
+  // by min_dist (>= 2^-20) either way, so 2^-96 may stand in for it.
+  float dist = kShort ? sqrt_short(fmaxf(dist_sq, 0x1p-96f)) : sqrtf(dist_sq);
   if (dist < wp.min_dist) dist = wp.min_dist;
   bool accept = leaf;
   if (!leaf) {
@@ -505,9 +574,17 @@ __device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, f
     accept = a < t || (wp.theta_tie_le && a == t);  // == (size / dist < theta)
   }
   if (accept) {
-    const float f = (wp.k * 1.0f * q) / (dist * dist);
-    ax += f * (dx / dist);
-    ay += f * (dy / dist);
+    if (kShort) {
+      const float d2 = dist * dist;
+      const float f = div_short<false>(wp.k * 1.0f * q, d2, recip_short(d2));  // k*q != 0
+      const float rd = recip_short(dist);
+      ax += f * div_short(dx, dist, rd);
+      ay += f * div_short(dy, dist, rd);
+    } else {
+      const float f = (wp.k * 1.0f * q) / (dist * dist);
+      ax += f * (dx / dist);
+      ay += f * (dy / dist);
+    }
     return true;
   }
   return false;
@@ -531,11 +608,11 @@ __device__ __forceinline__ int node_next(const BhNode& nd, int c) {
   return nd.w < 0 ? c + 1 : nd.z;
 }
 
-template <bool kStats, int W, bool kFast>
-__global__ __launch_bounds__(kBlock) void bh_traverse(
+template <bool kStats, int W, bool kFast, bool kShort>
+__device__ __forceinline__ void bh_walk_body(
     int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
     const float* __restrict__ y, const BhGeometry* __restrict__ g,
-    const BhNode* __restrict__ nodes, WalkParams wp, float* __restrict__ fx,
+    const BhNode* __restrict__ nodes, const WalkParams& wp, float* __restrict__ fx,
     float* __restrict__ fy, unsigned long long* __restrict__ stats) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = gid / W, r = gid % W;
@@ -559,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
     bool split = false;
     if (!(root_leaf && root.z == p)) {
       float rx = 0.0f, ry = 0.0f;
-      if (bh_node<kFast>(root, root_leaf, px, py, wp, size_root, rx, ry)) {
+      if (bh_node<kFast, kShort>(root, root_leaf, px, py, wp, size_root, rx, ry)) {
         if (r == 0) {
           qx[0] = rx;
           qy[0] = ry;
@@ -585,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
           const bool leaf = nd.w < 0;
           bool move_on = leaf && nd.z == p;
           if (!move_on) {
-            move_on = bh_node<kFast>(nd, leaf, px, py, wp, size_root, ax, ay);
+            move_on = bh_node<kFast, kShort>(nd, leaf, px, py, wp, size_root, ax, ay);
             if (kStats) {
               if (move_on) ++inter;
               else ++opened;
@@ -624,6 +701,20 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
   }
 }
 
+// The exact walk takes the short IEEE sequences when this iteration's
+// coordinates and the parameters allow them (a uniform branch).
+template <bool kStats, int W, bool kFast>
+__global__ __launch_bounds__(kBlock) void bh_traverse(
+    int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
+    const float* __restrict__ y, const BhGeometry* __restrict__ g,
+    const BhNode* __restrict__ nodes, WalkParams wp, float* __restrict__ fx,
+    float* __restrict__ fy, unsigned long long* __restrict__ stats) {
+  if (!kFast && wp.short_ok && g->safe_range)
+    bh_walk_body<kStats, W, kFast, true>(n, vals, x, y, g, nodes, wp, fx, fy, stats);
+  else
+    bh_walk_body<kStats, W, kFast, false>(n, vals, x, y, g, nodes, wp, fx, fy, stats);
+}
+
 template <bool kStats>
 int launch_walk(Job& j, unsigned long long* stats) {
   BhState& b = j.bh;
@@ -645,6 +736,10 @@ int launch_walk(Job& j, unsigned long long* stats) {
     uint32_t bits;
     std::memcpy(&bits, &th, 4);
     wp.theta_tie_le = (std::isfinite(th) && (bits & 1u)) ? 1 : 0;
+  }
+  {
+    const float md = j.p.min_dist, ak = std::fabs(j.p.repel_weight);
+    wp.short_ok = md >= 0x1p-20f && md <= 0x1p20f && ak >= 0x1p-40f && ak <= 0x1p10f;
   }
 #define FX_WALK_ARGS                                                                      \
   grid, block, 0, j.stream, j.N, b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(), \
@@ -682,7 +777,7 @@ int bh_alloc(Job& j) {
   b.hi_stride = ((n + 4095) / 4096 + 15) / 16 * 16 + 16;
   int rc;
   fx_ctx* c = j.ctx;
-  if ((rc = b.part.alloc(c, sizeof(float4) * kReduceBlocks)) ||
+  if ((rc = b.part.alloc(c, 2 * sizeof(float4) * kReduceBlocks)) ||
       (rc = b.geom.alloc(c, sizeof(BhGeometry))) || (rc = b.keys.alloc(c, 8 * n)) ||
       (rc = b.vals.alloc(c, 4 * n)) || (rc = b.lcp.alloc(c, pad16(n))) ||
       (rc = b.m1.alloc(c, pad16(c1))) || (rc = b.m2.alloc(c, pad16(b.nb1))) ||

@@ -16,6 +16,7 @@ struct BhGeometry {
   int32_t num_nodes;  // M of the last build
   int32_t overflow;   // set when M exceeded the node capacity
   float size_f;       // root cell size in FP32; a cell at depth d has size size_f * 2^-d
+  int32_t safe_range; // every coordinate is 0 or of magnitude in [2^-20, 2^20], none NaN
 };
 
 // Quadtree node in DFS pre-order, one 16-byte record (a single gather per

@@ -161,6 +161,31 @@ def test_barnes_hut_clustered():
     assert s["nodes"] == st["nodes"] and s["interactions"] == st["interactions"]
 
 
+@pytest.mark.parametrize("lo,hi,zeros", [(0.0, 3.0e6, False), (0.0, 1.0e-3, True),
+                                         (2.0 ** -20, 2.0 ** 20, True), (-2.0 ** 20, 2.0 ** 20, True)])
+def test_barnes_hut_coordinate_ranges(lo, hi, zeros):
+    # The exact walk uses short IEEE square-root/division sequences only when
+    # every coordinate is 0 or of magnitude in [2^-20, 2^20]; outside (3e6,
+    # values below 2^-20) it takes the general sequences. Both bit-identical.
+    print(f"case: BH range [{lo}, {hi}] zeros={zeros}", flush=True)
+    rng = np.random.default_rng(21)
+    N = 20000
+    x = rng.uniform(lo, hi, N).astype(np.float32)
+    y = rng.uniform(lo, hi, N).astype(np.float32)
+    if zeros:
+        x[::97] = 0.0
+        y[::89] = 0.0
+    if lo == 2.0 ** -20:
+        x[:50] = np.float32(2.0 ** -20)
+        y[50:100] = np.float32(2.0 ** 20)
+    circ = fxn.Circuit.from_nets(N, [[i, (i + 1) % N] for i in range(N)])
+    job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig())
+    job.set_positions(x, y)
+    gx, gy = job.forces(2)
+    rx, ry, st = orc.bh_record(x, y, nthreads=8)
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+
+
 @pytest.mark.parametrize("G", [32, 64, 128, 512])
 def test_density_fields_vs_fp64(G):
     print(f"case: DCT G={G}", flush=True)
