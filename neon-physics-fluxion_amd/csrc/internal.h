@@ -96,3 +96,44 @@ int fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
 constexpr int kReduceBlocks = 256;
 
 }  // namespace fx
+
+// ---- short IEEE sequences (shared by the Barnes-Hut walk and the leaf loop) ----
+namespace fx {
+// IEEE arithmetic on restricted operands, without the scaling and
+// special-value steps of the general sequences:
+//  * sqrt of x in [2^-96, 2^96]: the hardware estimate corrected by the
+//    residuals of its two neighbours (fma(-s', s, x) decides between s-1ulp,
+//    s and s+1ulp) — the general sequence for such x, which only adds a
+//    2^32 prescale for x < 2^-96 and returns 0, inf and NaN unchanged;
+//  * num/den for den of magnitude in [2^-40, 2^44] and num zero or of
+//    magnitude in [2^-100, 2^40]: the reciprocal estimate refined by one
+//    Newton step, then the quotient corrected twice by its exact residual
+//    fma(-den, q, num) — the general sequence without the operand scaling
+//    (never triggered for such operands) and the final special-case fixup
+//    (a no-op for normal quotients); a zero numerator gives itself.
+// The reciprocal depends on the divisor only, so two divisions by the same
+// divisor share it.
+__device__ __forceinline__ float sqrt_short(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __int_as_float(__float_as_int(s) - 1);
+  const float up = __int_as_float(__float_as_int(s) + 1);
+  const float s1 = fmaf(-dn, s, x) <= 0.0f ? dn : s;
+  return fmaf(-up, s, x) > 0.0f ? up : s1;
+}
+
+__device__ __forceinline__ float recip_short(float den) {
+  const float r = __builtin_amdgcn_rcpf(den);
+  const float e = fmaf(-den, r, 1.0f);
+  return fmaf(e, r, r);
+}
+
+template <bool kMaybeZero = true>
+__device__ __forceinline__ float div_short(float num, float den, float r) {
+  float q = num * r;
+  float e = fmaf(-den, q, num);
+  q = fmaf(e, r, q);
+  e = fmaf(-den, q, num);
+  q = fmaf(e, r, q);
+  return (kMaybeZero && num == 0.0f) ? num : q;
+}
+}  // namespace fx

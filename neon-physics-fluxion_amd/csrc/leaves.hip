@@ -36,6 +36,14 @@ constexpr int kMaxBlock = 256;
 constexpr int kMaxLeaf = 1024;
 constexpr int kMaxPerThread = 4;
 
+// A position of the range in which the exact pair force's square root and
+// divisions may use the short IEEE sequences: 0 or |v| in [2^-10, 2^20]
+// (with repel in [2^-20, 2^10] every operand then lies in their range).
+__device__ __forceinline__ bool in_short_range(float v) {
+  const float a = fabsf(v);
+  return v == 0.0f || (a >= 0x1p-10f && a <= 0x1p20f);
+}
+
 // Segment descriptor: (leaf, first lane in the workgroup, lanes, LDS offset).
 template <bool kFast>
 __global__ __launch_bounds__(kMaxBlock) void place_leaves_kernel(
@@ -43,11 +51,16 @@ __global__ __launch_bounds__(kMaxBlock) void place_leaves_kernel(
     const int* __restrict__ leaf_base, const int* __restrict__ leaf_n,
     const float4* __restrict__ region, const int* __restrict__ lptr,
     const int* __restrict__ lnb, float attract, float repel, float damping, float dt, int iters,
-    int init, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pvx,
-    float* __restrict__ pvy, float* __restrict__ pfx, float* __restrict__ pfy) {
+    int init, int short_ok, float* __restrict__ px, float* __restrict__ py,
+    float* __restrict__ pvx, float* __restrict__ pvy, float* __restrict__ pfx,
+    float* __restrict__ pfy) {
   __shared__ float sx[kMaxLeaf];
   __shared__ float sy[kMaxLeaf];
+  // Exact mode: 1 when some position of the workgroup's leaves lies outside
+  // the range of the short IEEE sequences (double-buffered per iteration).
+  __shared__ int out_of_range[2];
   const int tid = threadIdx.x;
+  if (tid < 2) out_of_range[tid] = short_ok ? 0 : 1;
   int leaf = -1, t = 0, T = 1, lo = 0;
   for (int s = wg_ptr[blockIdx.x]; s < wg_ptr[blockIdx.x + 1]; ++s) {
     const int4 d = segs[s];
@@ -85,8 +98,20 @@ __global__ __launch_bounds__(kMaxBlock) void place_leaves_kernel(
       sy[lo + i] = yr[k];
     }
   }
+  __syncthreads();
+  if (!kFast && short_ok) {
+#pragma unroll
+    for (int k = 0; k < kMaxPerThread; ++k) {
+      const int i = t + k * T;
+      if (leaf >= 0 && i < n && !(in_short_range(xr[k]) && in_short_range(yr[k])))
+        atomicOr(&out_of_range[0], 1);
+    }
+  }
   for (int it = 0; it < iters; ++it) {
     __syncthreads();
+    const int cur = it & 1;
+    const bool use_short = !kFast && !out_of_range[cur];
+    if (tid == 0 && !kFast && short_ok) out_of_range[cur ^ 1] = 0;
 #pragma unroll
     for (int k = 0; k < kMaxPerThread; ++k) {
       const int i = t + k * T;
@@ -117,6 +142,20 @@ __global__ __launch_bounds__(kMaxBlock) void place_leaves_kernel(
           const float f = repel * (r * r * r);
           ax = fmaf(f, dx, ax);
           ay = fmaf(f, dy, ay);
+        } else if (use_short) {
+          // The same operations through the short IEEE sequences (positions
+          // 0 or of magnitude in [2^-10, 2^20], repel in [2^-20, 2^10]):
+          // below 2^-96 the square root is replaced by 0.1 either way.
+          const float dist_sq = dx * dx + dy * dy;
+          const float dist = fmaxf(fx::sqrt_short(fmaxf(dist_sq, 0x1p-96f)), 0.1f);
+          const float d2 = dist * dist;
+          const float force = fx::div_short<false>(repel, d2, fx::recip_short(d2));
+          const float rd = fx::recip_short(dist);
+          const float fx_ij = fx::div_short(force * dx, dist, rd);
+          const float fy_ij = fx::div_short(force * dy, dist, rd);
+          const float nax = ax + fx_ij, nay = ay + fy_ij;
+          ax = (j != i) ? nax : ax;
+          ay = (j != i) ? nay : ay;
         } else {
           const float dist_sq = dx * dx + dy * dy;
           const float dist = fmaxf(sqrtf(dist_sq), 0.1f);
@@ -149,6 +188,8 @@ __global__ __launch_bounds__(kMaxBlock) void place_leaves_kernel(
       yr[k] = ny;
       sx[lo + i] = nx;
       sy[lo + i] = ny;
+      if (!kFast && short_ok && !(in_short_range(nx) && in_short_range(ny)))
+        atomicOr(&out_of_range[cur ^ 1], 1);
     }
   }
 #pragma unroll
@@ -335,20 +376,22 @@ int launch(fx_ctx* ctx, LeafBatch& lb, const fx_leaf_tile& t, const fx_params* p
   FX_HIP(hipMemcpyAsync(lb.dwg.p, wg_ptr.data(), 4 * wg_ptr.size(), hipMemcpyHostToDevice, st));
   FX_HIP(hipMemcpyAsync(lb.dsegs.p, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice, st));
   const dim3 grid((unsigned)*ntiles), block((unsigned)t.block);
+  const float ar = std::fabs(p->repel_weight);
+  const int short_ok = ar >= 0x1p-20f && ar <= 0x1p10f;
   FX_HIP(hipEventRecord(ctx->ev0, st));
   if (mode == FX_LEAF_FAST)
     hipLaunchKernelGGL(place_leaves_kernel<true>, grid, block, 0, st, lb.dwg.as<int>(),
                        lb.dsegs.as<int4>(), lb.dbase.as<int>(), lb.dn.as<int>(),
                        lb.dreg.as<float4>(), lb.dlptr.as<int>(), lb.dlnb.as<int>(),
                        p->attract_weight, p->repel_weight, p->damping, p->dt, iterations,
-                       init ? 1 : 0, lb.dx.as<float>(), lb.dy.as<float>(), lb.dvx.as<float>(),
+                       init ? 1 : 0, short_ok, lb.dx.as<float>(), lb.dy.as<float>(), lb.dvx.as<float>(),
                        lb.dvy.as<float>(), lb.dfx.as<float>(), lb.dfy.as<float>());
   else
     hipLaunchKernelGGL(place_leaves_kernel<false>, grid, block, 0, st, lb.dwg.as<int>(),
                        lb.dsegs.as<int4>(), lb.dbase.as<int>(), lb.dn.as<int>(),
                        lb.dreg.as<float4>(), lb.dlptr.as<int>(), lb.dlnb.as<int>(),
                        p->attract_weight, p->repel_weight, p->damping, p->dt, iterations,
-                       init ? 1 : 0, lb.dx.as<float>(), lb.dy.as<float>(), lb.dvx.as<float>(),
+                       init ? 1 : 0, short_ok, lb.dx.as<float>(), lb.dy.as<float>(), lb.dvx.as<float>(),
                        lb.dvy.as<float>(), lb.dfx.as<float>(), lb.dfy.as<float>());
   FX_HIP(hipGetLastError());
   FX_HIP(hipEventRecord(ctx->ev1, st));

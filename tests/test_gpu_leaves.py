@@ -46,6 +46,27 @@ def test_exact_large_leaves_and_edge_sizes(ctx):
     assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
 
 
+@pytest.mark.parametrize("cx,size,repel", [(5e-4, 8e-4, 0.5), (3.0e6, 100.0, 0.5), (50.0, 100.0, 1e-7),
+                                           (0.05, 0.1, 0.5)])
+def test_exact_outside_short_range(ctx, cx, size, repel):
+    # The exact loop takes short IEEE sequences only while every position of a
+    # workgroup's leaves is 0 or within [2^-10, 2^20] and repel within
+    # [2^-20, 2^10]; leaves near 0 (positions pass below 2^-10 during the
+    # run), far out (3e6) or a tiny repel use the general ones. Bit-identical.
+    print(f"case: exact range cx={cx} size={size} repel={repel}", flush=True)
+    circ, lp, gates, reg, aptr, adj = _batch(40, 555)
+    reg = reg.copy()
+    reg[:, 0] = cx
+    reg[:, 1] = cx
+    reg[:, 2] = size
+    reg[:, 3] = size
+    cfg = fxn.PlacerConfig(force_iterations=30, repel_weight=repel)
+    pl = fxn.Placement.new(circ.num_gates)
+    fxn.place_leaves(ctx, lp, gates, reg, circ, pl, cfg)
+    ox, oy, *_ = orc.place_leaves(lp, gates, reg, circ.num_gates, aptr, adj, repel=repel, iters=30)
+    assert np.array_equal(pl.gate_x, ox) and np.array_equal(pl.gate_y, oy)
+
+
 def test_permuted_gate_order(ctx):
     # leaf gate lists in arbitrary global order (global_to_local of placer.rs:231-235)
     print("case: permuted gates", flush=True)
