@@ -242,6 +242,33 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
   vals[i] = (uint32_t)i;
 }
 
+// After a stable sort on the top 32 key bits: the thread at the head of each
+// run of equal top halves (two or more keys) orders the run by (key, index)
+// with an insertion sort — linear for runs already in order (equal full keys,
+// e.g. co-located particles).
+__global__ __launch_bounds__(kBlock) void sort_low_halves(int n, uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i + 1 >= n) return;
+  const uint32_t top = (uint32_t)(keys[i] >> 32);
+  if ((uint32_t)(keys[i + 1] >> 32) != top) return;
+  if (i > 0 && (uint32_t)(keys[i - 1] >> 32) == top) return;
+  int e = i + 2;
+  while (e < n && (uint32_t)(keys[e] >> 32) == top) ++e;
+  for (int a = i + 1; a < e; ++a) {
+    const uint64_t k = keys[a];
+    const uint32_t v = vals[a];
+    int b = a - 1;
+    while (b >= i && (keys[b] > k || (keys[b] == k && vals[b] > v))) {
+      keys[b + 1] = keys[b];
+      vals[b + 1] = vals[b];
+      --b;
+    }
+    keys[b + 1] = k;
+    vals[b + 1] = v;
+  }
+}
+
 __device__ __forceinline__ int lcp_levels(const uint64_t* __restrict__ keys, int n, int a, int D) {
   if (a < 0 || a >= n - 1) return -1;
   const uint64_t x = keys[a] ^ keys[a + 1];
@@ -800,8 +827,15 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
   FX_HIP(hipGetLastError());
   if ((rc = phase_end(j, kPhaseBhKeys)) || (rc = phase_begin(j, kPhaseBhSort))) return rc;
-  if ((rc = radix_sort_pairs(j.ctx, st, b.sort, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), n, 0, 64)))
+  // Radix passes over the top 32 key bits only (16 tree levels), then runs
+  // of equal top halves — particles closer than the 2^-16 root fraction, rare
+  // — ordered by the full key in place: the result is the stable sort by the
+  // full key, as the indices enter in ascending order.
+  if ((rc = radix_sort_pairs(j.ctx, st, b.sort, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), n, 32, 64)))
     return rc;
+  hipLaunchKernelGGL(sort_low_halves, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
+                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
+  FX_HIP(hipGetLastError());
   if ((rc = phase_end(j, kPhaseBhSort)) || (rc = phase_begin(j, kPhaseBhTree))) return rc;
   hipLaunchKernelGGL(bh_lcp_count, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
