@@ -811,7 +811,8 @@ int bh_grow(Job& j, int64_t needed) {
 
 // extra_fill: an int32 array of extra_fill_n entries zeroed by the first
 // kernel (the density grid counts of the same iteration), or null.
-static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n = 0) {
+static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n = 0,
+                    Hook* after_sort = nullptr) {
   BhState& b = j.bh;
   const int n = j.N;
   hipStream_t st = j.stream;
@@ -836,7 +837,12 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
   hipLaunchKernelGGL(sort_low_halves, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
   FX_HIP(hipGetLastError());
-  if ((rc = phase_end(j, kPhaseBhSort)) || (rc = phase_begin(j, kPhaseBhTree))) return rc;
+  if ((rc = phase_end(j, kPhaseBhSort))) return rc;
+  if (after_sort) {
+    b.sorted_valid = true;
+    if ((rc = (*after_sort)())) return rc;
+  }
+  if ((rc = phase_begin(j, kPhaseBhTree))) return rc;
   hipLaunchKernelGGL(bh_lcp_count, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
                      b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), j.x.as<float>(), j.y.as<float>(),
                      (const BhGeometry*)g, b.lcp.as<int8_t>(), b.m1.as<int8_t>(), b.m2.as<int8_t>(),
@@ -859,8 +865,8 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
   return phase_end(j, kPhaseBhTree);
 }
 
-int bh_forces(Job& j, int32_t* zero_counts, int64_t zero_counts_n) {
-  int rc = bh_build(j, zero_counts, zero_counts_n);
+int bh_forces(Job& j, int32_t* zero_counts, int64_t zero_counts_n, Hook* after_sort) {
+  int rc = bh_build(j, zero_counts, zero_counts_n, after_sort);
   if (rc) return rc;
   BhState& b = j.bh;
   if ((rc = phase_begin(j, kPhaseBhWalk))) return rc;

@@ -365,10 +365,10 @@ static void parts3(const DevBuf& a, const DevBuf& b, const DevBuf& c, const uint
 }
 
 // rho -> phi_hat (t1) and E (ex, ey); U = C^T phi_hat is left in t0.
-int dct_field(Job& j, bool counts_cleared) {
+int dct_field(Job& j, bool counts_cleared, hipStream_t stream) {
   DctState& d = j.dct;
   const int G = d.G;
-  hipStream_t st = j.stream;
+  hipStream_t st = stream ? stream : j.stream;
   int rc;
   if (!counts_cleared && (rc = fill_i32(st, d.counts.as<int32_t>(), (int64_t)G * G, 0)))
     return rc;

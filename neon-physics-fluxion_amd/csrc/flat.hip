@@ -451,11 +451,31 @@ static int enqueue_phases(Job& j, int comps, bool integrate) {
   }
   // With both, the tree build's first kernel also clears the density counts.
   const bool shared_fill = (comps & 2) && (comps & 4);
+  // With both, the density field runs on the second stream from the key
+  // sort on (its deposit walks the sorted order), concurrently with the tree
+  // build and walk; not while the DCT phase itself is being timed.
+  const bool fork = shared_fill && j.aux && !(j.timing_mask & (1 << kPhaseDct));
+  struct Fork : Hook {
+    Job* j;
+    int operator()() override {
+      FX_HIP(hipEventRecord(j->ev_fork, j->stream));
+      FX_HIP(hipStreamWaitEvent(j->aux, j->ev_fork, 0));
+      int rc = dct_field(*j, true, j->aux);
+      if (rc) return rc;
+      FX_HIP(hipEventRecord(j->ev_join, j->aux));
+      return FX_OK;
+    }
+  } forked;
+  forked.j = &j;
   if (comps & 2) {
     const int64_t G = j.p.density_grid;
-    if ((rc = bh_forces(j, shared_fill ? j.dct.counts.as<int32_t>() : nullptr, G * G))) return rc;
+    if ((rc = bh_forces(j, shared_fill ? j.dct.counts.as<int32_t>() : nullptr, G * G,
+                        fork ? &forked : nullptr)))
+      return rc;
   }
-  if (comps & 4) {
+  if (fork) {
+    FX_HIP(hipStreamWaitEvent(j.stream, j.ev_join, 0));
+  } else if (comps & 4) {
     if ((rc = phase_begin(j, kPhaseDct)) || (rc = dct_field(j, shared_fill)) ||
         (rc = phase_end(j, kPhaseDct)))
       return rc;
@@ -680,6 +700,14 @@ extern "C" int fx_job_create(fx_ctx* ctx, const fx_netlist* net, const fx_params
     delete job;
     return fx::fail(FX_E_DEVICE, "stream creation failed");
   }
+  if (G > 0 && !(params->flags & FX_FLAG_NO_REPULSION) && !(params->flags & FX_FLAG_NO_DENSITY)) {
+    if (hipStreamCreateWithFlags(&j.aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&j.ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&j.ev_join, hipEventDisableTiming) != hipSuccess) {
+      fx_job_destroy(job);
+      return fx::fail(FX_E_DEVICE, "stream creation failed");
+    }
+  }
   int rc = job_upload(j, net);
   if (!rc) rc = fx::bh_alloc(j);
   if (!rc && G > 0) rc = fx::dct_alloc(j);
@@ -696,10 +724,15 @@ extern "C" void fx_job_destroy(fx_job* job) {
   if (!job) return;
   Job& j = job->j;
   if (j.stream) (void)hipStreamSynchronize(j.stream);
+  if (j.aux) (void)hipStreamSynchronize(j.aux);
   fx::drop_graph(j);
   fx::drop_timing(j);
-  hipStream_t s = j.stream;
+  hipStream_t s = j.stream, a = j.aux;
+  hipEvent_t e0 = j.ev_fork, e1 = j.ev_join;
   delete job;  // DevBufs release their memory
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipStreamDestroy(a);
   if (s) (void)hipStreamDestroy(s);
 }
 

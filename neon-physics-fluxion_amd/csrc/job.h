@@ -61,6 +61,11 @@ struct DctState {
 struct Job {
   fx_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
+  // Second stream of the iteration: the density field (deposit + DCT) runs
+  // on it while the tree is built and walked on `stream` (fork after the
+  // key sort, join before the combine step).
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int32_t N = 0, E = 0, P = 0;
   float die_w = 0.f, die_h = 0.f;
   fx_params p{};
@@ -105,10 +110,17 @@ int phase_end(Job& j, int phase);
 int springs_centroids(Job& j);
 // Writes j.bh.fx/fy (per original index). zero_counts (may be null): an int32
 // array of zero_counts_n entries cleared by the build's first kernel.
-int bh_forces(Job& j, int32_t* zero_counts = nullptr, int64_t zero_counts_n = 0);
-// Writes the field; counts_cleared: the point counts were already zeroed
-// earlier in the iteration (by bh_forces).
-int dct_field(Job& j, bool counts_cleared = false);
+// after_sort (may be null): called once the key sort is enqueued, before the
+// tree build (the sorted order is then valid for work forked from there).
+struct Hook {
+  virtual int operator()() = 0;
+  virtual ~Hook() = default;
+};
+int bh_forces(Job& j, int32_t* zero_counts = nullptr, int64_t zero_counts_n = 0,
+              Hook* after_sort = nullptr);
+// Writes the field on stream st (0: the job stream); counts_cleared: the
+// point counts were already zeroed earlier in the iteration (by bh_forces).
+int dct_field(Job& j, bool counts_cleared = false, hipStream_t st = nullptr);
 int combine(Job& j, int components, bool integrate);  // forces -> (fxo,fyo) or integrate
 int bh_alloc(Job& j);
 int dct_alloc(Job& j);
