@@ -372,9 +372,10 @@ int fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
   return FX_OK;
 }
 
-int springs_centroids(Job& j) {
+int springs_centroids(Job& j, hipStream_t st) {
   if (j.E == 0) return FX_OK;
-  hipLaunchKernelGGL(net_centroid_kernel, dim3(div_up(j.E, kBlock)), dim3(kBlock), 0, j.stream,
+  hipLaunchKernelGGL(net_centroid_kernel, dim3(div_up(j.E, kBlock)), dim3(kBlock), 0,
+                     st ? st : j.stream,
                      j.E, j.net_ptr.as<int>(), j.net_pins.as<int>(), j.x.as<float>(),
                      j.y.as<float>(), j.p.attract_weight, j.netc.as<float4>());
   FX_HIP(hipGetLastError());
@@ -442,19 +443,28 @@ int phase_end(Job& j, int phase) {
   return FX_OK;
 }
 
+constexpr int32_t kForkMinGates = 65536;
+
 static int enqueue_phases(Job& j, int comps, bool integrate) {
   int rc;
-  if (comps & 1) {
-    if ((rc = phase_begin(j, kPhaseSprings)) || (rc = springs_centroids(j)) ||
-        (rc = phase_end(j, kPhaseSprings)))
-      return rc;
-  }
   // With both, the tree build's first kernel also clears the density counts.
   const bool shared_fill = (comps & 2) && (comps & 4);
-  // With both, the density field runs on the second stream from the key
-  // sort on (its deposit walks the sorted order), concurrently with the tree
-  // build and walk; not while the DCT phase itself is being timed.
-  const bool fork = shared_fill && j.aux && !(j.timing_mask & (1 << kPhaseDct));
+  // With both, the net centroids and then, from the key sort on (its deposit
+  // walks the sorted order), the density field run on the second stream,
+  // concurrently with the tree build and walk; not while those phases are
+  // being timed.
+  const bool fork = shared_fill && j.aux &&
+                    !(j.timing_mask & ((1 << kPhaseDct) | (1 << kPhaseSprings)));
+  if (comps & 1) {
+    if (fork) {
+      FX_HIP(hipEventRecord(j.ev_start, j.stream));
+      FX_HIP(hipStreamWaitEvent(j.aux, j.ev_start, 0));
+      if ((rc = springs_centroids(j, j.aux))) return rc;
+    } else if ((rc = phase_begin(j, kPhaseSprings)) || (rc = springs_centroids(j)) ||
+               (rc = phase_end(j, kPhaseSprings))) {
+      return rc;
+    }
+  }
   struct Fork : Hook {
     Job* j;
     int operator()() override {
@@ -700,8 +710,13 @@ extern "C" int fx_job_create(fx_ctx* ctx, const fx_netlist* net, const fx_params
     delete job;
     return fx::fail(FX_E_DEVICE, "stream creation failed");
   }
-  if (G > 0 && !(params->flags & FX_FLAG_NO_REPULSION) && !(params->flags & FX_FLAG_NO_DENSITY)) {
+  // The second stream pays off once the concurrent work outweighs a fork and
+  // a join per iteration (measured: slower below ~10^4 gates, where the
+  // kernels are a few microseconds).
+  if (G > 0 && j.N >= fx::kForkMinGates && !(params->flags & FX_FLAG_NO_REPULSION) &&
+      !(params->flags & FX_FLAG_NO_DENSITY)) {
     if (hipStreamCreateWithFlags(&j.aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&j.ev_start, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&j.ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&j.ev_join, hipEventDisableTiming) != hipSuccess) {
       fx_job_destroy(job);
@@ -728,8 +743,9 @@ extern "C" void fx_job_destroy(fx_job* job) {
   fx::drop_graph(j);
   fx::drop_timing(j);
   hipStream_t s = j.stream, a = j.aux;
-  hipEvent_t e0 = j.ev_fork, e1 = j.ev_join;
+  hipEvent_t e0 = j.ev_fork, e1 = j.ev_join, e2 = j.ev_start;
   delete job;  // DevBufs release their memory
+  if (e2) (void)hipEventDestroy(e2);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (a) (void)hipStreamDestroy(a);

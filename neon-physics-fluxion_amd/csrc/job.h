@@ -61,11 +61,12 @@ struct DctState {
 struct Job {
   fx_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
-  // Second stream of the iteration: the density field (deposit + DCT) runs
-  // on it while the tree is built and walked on `stream` (fork after the
-  // key sort, join before the combine step).
+  // Second stream of the iteration: the net centroids (fork at the start)
+  // and the density field (deposit + DCT, fork after the key sort) run on it
+  // while the keys are sorted and the tree is built and walked on `stream`
+  // (join before the combine step).
   hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_start = nullptr, ev_fork = nullptr, ev_join = nullptr;
   int32_t N = 0, E = 0, P = 0;
   float die_w = 0.f, die_h = 0.f;
   fx_params p{};
@@ -107,7 +108,7 @@ int phase_begin(Job& j, int phase);
 int phase_end(Job& j, int phase);
 
 // Phases (each enqueues on job.stream; no host synchronisation).
-int springs_centroids(Job& j);
+int springs_centroids(Job& j, hipStream_t st = nullptr);
 // Writes j.bh.fx/fy (per original index). zero_counts (may be null): an int32
 // array of zero_counts_n entries cleared by the build's first kernel.
 // after_sort (may be null): called once the key sort is enqueued, before the
