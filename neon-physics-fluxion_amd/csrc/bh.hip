@@ -530,53 +530,72 @@ struct WalkParams {
   int short_ok;
 };
 
-// Per-node interaction of the walk. Exact: the reference's operations with
-// IEEE square root and division (bit-identical to the FP32 path of record;
-// the opening test is the same predicate evaluated exactly in FP64).
-// Fast: the same law with v_rsq_f32 and FMA and the opening test squared
-// (size^2 < theta^2 max(d^2, min_dist^2)); forces within 1e-6 relative.
-template <bool kFast, bool kShort = false>
-__device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, float py,
-                                        const WalkParams& wp, float size_root, float& ax,
-                                        float& ay) {
-  const float dx = px - nd.x, dy = py - nd.y;
-  const float q = (float)(nd.w & (leaf ? 0x7FFFFFFF : kBhMaxCount));
+// A node visit in two parts: the opening decision (with what the force
+// needs: dx, dy and the clamped distance, or its square in fast mode), then
+// the force. The walk issues the next node's load between the two.
+struct NodeTerm {
+  float dx, dy, d;  // d: distance after the min_dist clamp (fast: its square)
+  float q;
+};
+
+template <bool kFast, bool kShort>
+__device__ __forceinline__ bool bh_decide(const BhNode& nd, bool leaf, float px, float py,
+                                          const WalkParams& wp, float size_root, NodeTerm& t) {
+  t.dx = px - nd.x;
+  t.dy = py - nd.y;
+  t.q = (float)(nd.w & (leaf ? 0x7FFFFFFF : kBhMaxCount));
   const float size = ldexpf(size_root, -((nd.w >> 26) & 31));
   if (kFast) {
-    const float d2 = fmaxf(fmaf(dx, dx, dy * dy), wp.min_dist * wp.min_dist);
-    if (!leaf && !(size * size < wp.theta * wp.theta * d2)) return false;
-    const float r = __builtin_amdgcn_rsqf(d2);
-    const float f = wp.k * q * (r * r * r);
-    ax = fmaf(f, dx, ax);
-    ay = fmaf(f, dy, ay);
-    return true;
+    t.d = fmaxf(fmaf(t.dx, t.dx, t.dy * t.dy), wp.min_dist * wp.min_dist);
+    return leaf || size * size < wp.theta * wp.theta * t.d;
   }
-  const float dist_sq = dx * dx + dy * dy;
+  const float dist_sq = t.dx * t.dx + t.dy * t.dy;
   // Short form: below 2^-96 the square root is under 2^-48 and is replaced
   // by min_dist (>= 2^-20) either way, so 2^-96 may stand in for it.
   float dist = kShort ? sqrt_short(fmaxf(dist_sq, 0x1p-96f)) : sqrtf(dist_sq);
   if (dist < wp.min_dist) dist = wp.min_dist;
-  bool accept = leaf;
-  if (!leaf) {
-    const double t = wp.theta_m * (double)dist;
-    const double a = (double)size;
-    accept = a < t || (wp.theta_tie_le && a == t);  // == (size / dist < theta)
+  t.d = dist;
+  if (leaf) return true;
+  const double th = wp.theta_m * (double)dist;
+  const double a = (double)size;
+  return a < th || (wp.theta_tie_le && a == th);  // == (size / dist < theta)
+}
+
+template <bool kFast, bool kShort>
+__device__ __forceinline__ void bh_force(const NodeTerm& t, const WalkParams& wp, float& ax,
+                                         float& ay) {
+  if (kFast) {
+    const float r = __builtin_amdgcn_rsqf(t.d);
+    const float f = wp.k * t.q * (r * r * r);
+    ax = fmaf(f, t.dx, ax);
+    ay = fmaf(f, t.dy, ay);
+  } else if (kShort) {
+    const float dist = t.d, d2 = dist * dist;
+    const float f = div_short<false>(wp.k * 1.0f * t.q, d2, recip_short(d2));  // k*q != 0
+    const float rd = recip_short(dist);
+    ax += f * div_short(t.dx, dist, rd);
+    ay += f * div_short(t.dy, dist, rd);
+  } else {
+    const float dist = t.d;
+    const float f = (wp.k * 1.0f * t.q) / (dist * dist);
+    ax += f * (t.dx / dist);
+    ay += f * (t.dy / dist);
   }
-  if (accept) {
-    if (kShort) {
-      const float d2 = dist * dist;
-      const float f = div_short<false>(wp.k * 1.0f * q, d2, recip_short(d2));  // k*q != 0
-      const float rd = recip_short(dist);
-      ax += f * div_short(dx, dist, rd);
-      ay += f * div_short(dy, dist, rd);
-    } else {
-      const float f = (wp.k * 1.0f * q) / (dist * dist);
-      ax += f * (dx / dist);
-      ay += f * (dy / dist);
-    }
-    return true;
-  }
-  return false;
+}
+
+// Exact: the reference's operations with IEEE square root and division
+// (bit-identical to the FP32 path of record; the opening test is the same
+// predicate evaluated exactly in FP64). Fast: the same law with v_rsq_f32
+// and FMA and the opening test squared (size^2 < theta^2 max(d^2,
+// min_dist^2)); forces within 1e-6 relative.
+template <bool kFast, bool kShort = false>
+__device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, float py,
+                                        const WalkParams& wp, float size_root, float& ax,
+                                        float& ay) {
+  NodeTerm t;
+  if (!bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, t)) return false;
+  bh_force<kFast, kShort>(t, wp, ax, ay);
+  return true;
 }
 
 // One 16-byte load of a node (a single vector load: the compiler must not
@@ -646,24 +665,29 @@ __device__ __forceinline__ void bh_walk_body(
         int cur = c;
         const int end = node_next(nodes[c], c);
         float ax = 0.0f, ay = 0.0f;
-        while (cur < end) {
-          const BhNode nd = load_node(nodes, cur);
+        BhNode nd = load_node(nodes, cur);
+        while (true) {
           const bool leaf = nd.w < 0;
-          bool move_on = leaf && nd.z == p;
-          if (!move_on) {
-            move_on = bh_node<kFast, kShort>(nd, leaf, px, py, wp, size_root, ax, ay);
-            if (kStats) {
-              if (move_on) ++inter;
-              else ++opened;
-            }
+          const bool self = leaf && nd.z == p;
+          NodeTerm tm;
+          const bool accept =
+              !self && bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, tm);
+          if (kStats && !self) {
+            if (accept) ++inter;
+            else ++opened;
           }
-          if (move_on) {
-            const int nxt = leaf ? cur + 1 : nd.z;
-            if (nxt <= cur) break;  // malformed tree: never loop
-            cur = nxt;
-          } else {
-            cur = cur + 1;
+          const bool move_on = self || accept;
+          const int nxt = move_on ? (leaf ? cur + 1 : nd.z) : cur + 1;
+          if (nxt <= cur || nxt >= end) {  // done (or a malformed tree: never loop)
+            if (accept) bh_force<kFast, kShort>(tm, wp, ax, ay);
+            break;
           }
+          // The next node's load is in flight while this node's force is
+          // computed.
+          const BhNode nn = load_node(nodes, nxt);
+          if (accept) bh_force<kFast, kShort>(tm, wp, ax, ay);
+          cur = nxt;
+          nd = nn;
         }
         qx[q] = ax;
         qy[q] = ay;
