@@ -63,6 +63,12 @@ typedef struct fx_params {
  * the reference's IEEE division/sqrt sequence: forces within 1e-6 relative of
  * the exact walk (which is bit-identical to the FP32 CPU path of record). */
 #define FX_FLAG_FAST_MATH 8
+/* Keep every phase of a job on its one stream. By default a job of 65,536
+ * gates or more runs the net centroids and the density field on a second
+ * stream, concurrently with the Barnes-Hut tree build and walk (same
+ * results); callers that overlap many jobs themselves (the placement server
+ * does) set this to keep the stream count down. */
+#define FX_FLAG_ONE_STREAM 16
 
 /* Fills PlacerConfig::default() and the flat-loop defaults
  * (theta 0.5, min_dist 0.1, density weight 1.0, target 0.8, grid 0). */

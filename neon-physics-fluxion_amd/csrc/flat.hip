@@ -714,7 +714,7 @@ extern "C" int fx_job_create(fx_ctx* ctx, const fx_netlist* net, const fx_params
   // a join per iteration (measured: slower below ~10^4 gates, where the
   // kernels are a few microseconds).
   if (G > 0 && j.N >= fx::kForkMinGates && !(params->flags & FX_FLAG_NO_REPULSION) &&
-      !(params->flags & FX_FLAG_NO_DENSITY)) {
+      !(params->flags & FX_FLAG_NO_DENSITY) && !(params->flags & FX_FLAG_ONE_STREAM)) {
     if (hipStreamCreateWithFlags(&j.aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&j.ev_start, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&j.ev_fork, hipEventDisableTiming) != hipSuccess ||

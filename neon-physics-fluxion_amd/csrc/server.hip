@@ -130,7 +130,10 @@ void fx_server::run() {
     for (Request* r : snapshot) {
       if (!r->job) {
         fx_netlist net{r->N, r->E, r->net_ptr.data(), r->net_pins.data(), r->die_w, r->die_h};
-        int rc = fx_job_create(ctx, &net, &r->params, &r->job);
+        // The server overlaps its requests itself: one stream per job.
+        fx_params prm = r->params;
+        prm.flags |= FX_FLAG_ONE_STREAM;
+        int rc = fx_job_create(ctx, &net, &prm, &r->job);
         if (rc == FX_OK) rc = fx_job_set_positions(r->job, r->x.data(), r->y.data(), r->N);
         if (rc == FX_OK && cfg.use_graphs) rc = fx_job_use_graph(r->job, 1);
         if (rc != FX_OK) {

@@ -24,6 +24,7 @@ FX_FLAG_NO_SPRINGS = 1
 FX_FLAG_NO_REPULSION = 2
 FX_FLAG_NO_DENSITY = 4
 FX_FLAG_FAST_MATH = 8
+FX_FLAG_ONE_STREAM = 16
 
 FX_LEAF_EXACT = 0
 FX_LEAF_FAST = 1
