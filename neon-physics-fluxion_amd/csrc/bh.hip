@@ -35,6 +35,7 @@
 #include <cstring>
 
 #include "job.h"
+#include "walk_open.h"
 
 namespace fx {
 
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
                                                   const float4* __restrict__ part, int nparts,
                                                   BhGeometry* __restrict__ gout,
                                                   uint64_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ vals) {
+                                                  uint32_t* __restrict__ vals, OpenRule rule) {
   __shared__ BhGeometry sg;
   if (threadIdx.x < 64) {
     BhGeometry gl;
@@ -217,6 +218,14 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
     }
   }
   __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    // The exact walk's opening thresholds, one per cell depth.
+    int ok = 1;
+    const float s = open_threshold_sq(ldexpf(sg.size_f, -(int)(threadIdx.x & 31)), rule, &ok);
+    if (threadIdx.x < 32) gout->open_sq[threadIdx.x] = s;
+    const int all_ok = __all(ok);
+    if (threadIdx.x == 0) gout->open_ok = all_ok;
+  }
   const BhGeometry* g = &sg;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -540,19 +549,24 @@ struct NodeTerm {
 
 template <bool kFast, bool kShort>
 __device__ __forceinline__ bool bh_decide(const BhNode& nd, bool leaf, float px, float py,
-                                          const WalkParams& wp, float size_root, NodeTerm& t) {
+                                          const WalkParams& wp, float size_root,
+                                          const float* open_sq, NodeTerm& t) {
   t.dx = px - nd.x;
   t.dy = py - nd.y;
   t.q = (float)(nd.w & (leaf ? 0x7FFFFFFF : kBhMaxCount));
+  if (kShort) {
+    // The exact test as a threshold on the squared distance (walk_open.h);
+    // the force takes the square root of accepted cells only.
+    t.d = t.dx * t.dx + t.dy * t.dy;
+    return leaf || t.d >= open_sq[(nd.w >> 26) & 31];
+  }
   const float size = ldexpf(size_root, -((nd.w >> 26) & 31));
   if (kFast) {
     t.d = fmaxf(fmaf(t.dx, t.dx, t.dy * t.dy), wp.min_dist * wp.min_dist);
     return leaf || size * size < wp.theta * wp.theta * t.d;
   }
   const float dist_sq = t.dx * t.dx + t.dy * t.dy;
-  // Short form: below 2^-96 the square root is under 2^-48 and is replaced
-  // by min_dist (>= 2^-20) either way, so 2^-96 may stand in for it.
-  float dist = kShort ? sqrt_short(fmaxf(dist_sq, 0x1p-96f)) : sqrtf(dist_sq);
+  float dist = sqrtf(dist_sq);
   if (dist < wp.min_dist) dist = wp.min_dist;
   t.d = dist;
   if (leaf) return true;
@@ -570,11 +584,18 @@ __device__ __forceinline__ void bh_force(const NodeTerm& t, const WalkParams& wp
     ax = fmaf(f, t.dx, ax);
     ay = fmaf(f, t.dy, ay);
   } else if (kShort) {
-    const float dist = t.d, d2 = dist * dist;
+    // Below 2^-96 the square root is under 2^-48 and is replaced by min_dist
+    // (>= 2^-20) either way, so 2^-96 may stand in for it.
+    float dist = sqrt_short(fmaxf(t.d, 0x1p-96f));
+    if (dist < wp.min_dist) dist = wp.min_dist;
+    const float d2 = dist * dist;
     const float f = div_short<false>(wp.k * 1.0f * t.q, d2, recip_short(d2));  // k*q != 0
     const float rd = recip_short(dist);
-    ax += f * div_short(t.dx, dist, rd);
-    ay += f * div_short(t.dy, dist, rd);
+    // A zero dx gives +0 here where IEEE division gives a zero of dx's sign;
+    // the sums never tell them apart: they start at +0 and a sum is -0 only
+    // when both terms are -0, so they never hold -0, and x + (+-0) == x.
+    ax += f * div_short<false>(t.dx, dist, rd);
+    ay += f * div_short<false>(t.dy, dist, rd);
   } else {
     const float dist = t.d;
     const float f = (wp.k * 1.0f * t.q) / (dist * dist);
@@ -590,10 +611,10 @@ __device__ __forceinline__ void bh_force(const NodeTerm& t, const WalkParams& wp
 // min_dist^2)); forces within 1e-6 relative.
 template <bool kFast, bool kShort = false>
 __device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, float py,
-                                        const WalkParams& wp, float size_root, float& ax,
-                                        float& ay) {
+                                        const WalkParams& wp, float size_root,
+                                        const float* open_sq, float& ax, float& ay) {
   NodeTerm t;
-  if (!bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, t)) return false;
+  if (!bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, open_sq, t)) return false;
   bh_force<kFast, kShort>(t, wp, ax, ay);
   return true;
 }
@@ -620,8 +641,8 @@ template <bool kStats, int W, bool kFast, bool kShort>
 __device__ __forceinline__ void bh_walk_body(
     int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
     const float* __restrict__ y, const BhGeometry* __restrict__ g,
-    const BhNode* __restrict__ nodes, const WalkParams& wp, float* __restrict__ fx,
-    float* __restrict__ fy, unsigned long long* __restrict__ stats) {
+    const BhNode* __restrict__ nodes, const WalkParams& wp, const float* open_sq,
+    float* __restrict__ fx, float* __restrict__ fy, unsigned long long* __restrict__ stats) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = gid / W, r = gid % W;
   const bool valid = t < n;
@@ -644,7 +665,7 @@ __device__ __forceinline__ void bh_walk_body(
     bool split = false;
     if (!(root_leaf && root.z == p)) {
       float rx = 0.0f, ry = 0.0f;
-      if (bh_node<kFast, kShort>(root, root_leaf, px, py, wp, size_root, rx, ry)) {
+      if (bh_node<kFast, kShort>(root, root_leaf, px, py, wp, size_root, open_sq, rx, ry)) {
         if (r == 0) {
           qx[0] = rx;
           qy[0] = ry;
@@ -671,7 +692,7 @@ __device__ __forceinline__ void bh_walk_body(
           const bool self = leaf && nd.z == p;
           NodeTerm tm;
           const bool accept =
-              !self && bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, tm);
+              !self && bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, open_sq, tm);
           if (kStats && !self) {
             if (accept) ++inter;
             else ++opened;
@@ -722,10 +743,30 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
     const float* __restrict__ y, const BhGeometry* __restrict__ g,
     const BhNode* __restrict__ nodes, WalkParams wp, float* __restrict__ fx,
     float* __restrict__ fy, unsigned long long* __restrict__ stats) {
-  if (!kFast && wp.short_ok && g->safe_range)
-    bh_walk_body<kStats, W, kFast, true>(n, vals, x, y, g, nodes, wp, fx, fy, stats);
+  __shared__ float open_sq[32];
+  if (threadIdx.x < 32) open_sq[threadIdx.x] = g->open_sq[threadIdx.x];
+  __syncthreads();
+  if (!kFast && wp.short_ok && g->safe_range && g->open_ok)
+    bh_walk_body<kStats, W, kFast, true>(n, vals, x, y, g, nodes, wp, open_sq, fx, fy, stats);
   else
-    bh_walk_body<kStats, W, kFast, false>(n, vals, x, y, g, nodes, wp, fx, fy, stats);
+    bh_walk_body<kStats, W, kFast, false>(n, vals, x, y, g, nodes, wp, open_sq, fx, fy, stats);
+}
+
+// Exact form of size/dist < theta: theta's rounding boundary (the midpoint
+// of theta and its predecessor; for +inf, the FP32 overflow threshold; exact
+// in FP64) and the tie rule (a tie rounds to the even neighbour: down when
+// theta's significand is odd).
+OpenRule open_rule(const fx_params& p) {
+  OpenRule r;
+  const float th = p.theta;
+  const float lo = std::nextafterf(th, -INFINITY);
+  const double hi = (std::isinf(th) && th > 0) ? std::ldexp(1.0, 128) : (double)th;
+  r.theta_m = ((double)lo + hi) / 2;
+  uint32_t bits;
+  std::memcpy(&bits, &th, 4);
+  r.tie_le = (std::isfinite(th) && (bits & 1u)) ? 1 : 0;
+  r.min_dist = p.min_dist;
+  return r;
 }
 
 template <bool kStats>
@@ -738,18 +779,9 @@ int launch_walk(Job& j, unsigned long long* stats) {
   wp.k = j.p.repel_weight;
   wp.theta = j.p.theta;
   wp.min_dist = j.p.min_dist;
-  {
-    // Rounding boundary below theta: the midpoint of theta and its
-    // predecessor (for +inf, the FP32 overflow threshold); exact in FP64.
-    const float th = j.p.theta;
-    const float lo = std::nextafterf(th, -INFINITY);
-    const double hi = (std::isinf(th) && th > 0) ? std::ldexp(1.0, 128) : (double)th;
-    wp.theta_m = ((double)lo + hi) / 2;
-    // A tie rounds to the even neighbour: down when theta's significand is odd.
-    uint32_t bits;
-    std::memcpy(&bits, &th, 4);
-    wp.theta_tie_le = (std::isfinite(th) && (bits & 1u)) ? 1 : 0;
-  }
+  const OpenRule rule = open_rule(j.p);
+  wp.theta_m = rule.theta_m;
+  wp.theta_tie_le = rule.tie_le;
   {
     const float md = j.p.min_dist, ak = std::fabs(j.p.repel_weight);
     wp.short_ok = md >= 0x1p-20f && md <= 0x1p20f && ak >= 0x1p-40f && ak <= 0x1p10f;
@@ -849,7 +881,7 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
                      extra_fill ? extra_fill_n : 0, 0);
   hipLaunchKernelGGL(bh_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n, j.x.as<float>(),
                      j.y.as<float>(), (const float4*)b.part.as<float4>(), kReduceBlocks, g,
-                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
+                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), open_rule(j.p));
   FX_HIP(hipGetLastError());
   if ((rc = phase_end(j, kPhaseBhKeys)) || (rc = phase_begin(j, kPhaseBhSort))) return rc;
   // Radix passes over the top 32 key bits only (16 tree levels), then runs

@@ -17,6 +17,8 @@ struct BhGeometry {
   int32_t overflow;   // set when M exceeded the node capacity
   float size_f;       // root cell size in FP32; a cell at depth d has size size_f * 2^-d
   int32_t safe_range; // every coordinate is 0 or of magnitude in [2^-20, 2^20], none NaN
+  int32_t open_ok;    // open_sq holds the exact walk's thresholds
+  float open_sq[32];  // a cell at depth d is accepted iff d2 >= open_sq[d] (walk_open.h)
 };
 
 // Quadtree node in DFS pre-order, one 16-byte record (a single gather per
