@@ -556,9 +556,12 @@ __device__ __forceinline__ bool bh_decide(const BhNode& nd, bool leaf, float px,
   t.q = (float)(nd.w & (leaf ? 0x7FFFFFFF : kBhMaxCount));
   if (kShort) {
     // The exact test as a threshold on the squared distance (walk_open.h);
-    // the force takes the square root of accepted cells only.
+    // the force takes the square root of accepted cells only. The threshold
+    // is read for leaves too (an index in range), so its LDS read issues
+    // with the node and no branch splits the wave.
+    const float thr = open_sq[(nd.w >> 26) & 31];
     t.d = t.dx * t.dx + t.dy * t.dy;
-    return leaf || t.d >= open_sq[(nd.w >> 26) & 31];
+    return leaf | (t.d >= thr);
   }
   const float size = ldexpf(size_root, -((nd.w >> 26) & 31));
   if (kFast) {
@@ -692,7 +695,7 @@ __device__ __forceinline__ void bh_walk_body(
           const bool self = leaf && nd.z == p;
           NodeTerm tm;
           const bool accept =
-              !self && bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, open_sq, tm);
+              bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, open_sq, tm) & !self;
           if (kStats && !self) {
             if (accept) ++inter;
             else ++opened;
@@ -735,6 +738,90 @@ __device__ __forceinline__ void bh_walk_body(
   }
 }
 
+// One lane per particle: a single loop over the pre-order range below the
+// root. Each quadrant's partial sum is folded into the total when its subtree
+// ends, total = ((q0 + q1) + q2) + q3 as above (a partial sum is never -0:
+// it starts at +0, and a sum is -0 only when both terms are), so a wave never
+// waits at a quadrant boundary for its slowest lane.
+template <bool kStats, bool kFast, bool kShort>
+__device__ __forceinline__ void bh_walk_one(
+    int n, const uint32_t* __restrict__ vals, const float* __restrict__ x,
+    const float* __restrict__ y, const BhGeometry* __restrict__ g,
+    const BhNode* __restrict__ nodes, const WalkParams& wp, const float* open_sq,
+    float* __restrict__ fx, float* __restrict__ fy, unsigned long long* __restrict__ stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < n;
+  const int p = valid ? (int)vals[t] : -1;
+  if (g->overflow) {
+    if (valid) {
+      fx[p] = 0.0f;
+      fy[p] = 0.0f;
+    }
+    return;
+  }
+  float sx = 0.0f, sy = 0.0f;
+  unsigned long long inter = 0, opened = 0;
+  if (valid) {
+    const float px = x[p], py = y[p];
+    const float size_root = g->size_f;
+    const BhNode root = load_node(nodes, 0);
+    const bool root_leaf = root.w < 0;
+    const int root_skip = root_leaf ? 1 : root.z;
+    bool split = false;
+    if (!(root_leaf && root.z == p)) {
+      if (bh_node<kFast, kShort>(root, root_leaf, px, py, wp, size_root, open_sq, sx, sy)) {
+        if (kStats) ++inter;
+      } else {
+        split = true;
+        if (kStats) ++opened;
+      }
+    }
+    if (split && root_skip > 1) {
+      int cur = 1;
+      BhNode nd = load_node(nodes, cur);
+      int qend = node_next(nd, cur);  // end of the current quadrant's subtree
+      float ax = 0.0f, ay = 0.0f;
+      while (true) {
+        if (cur == qend) {  // the next quadrant's first node
+          sx = sx + ax;
+          sy = sy + ay;
+          ax = 0.0f;
+          ay = 0.0f;
+          qend = node_next(nd, cur);
+        }
+        const bool leaf = nd.w < 0;
+        const bool self = leaf && nd.z == p;
+        NodeTerm tm;
+        const bool accept =
+            bh_decide<kFast, kShort>(nd, leaf, px, py, wp, size_root, open_sq, tm) & !self;
+        if (kStats && !self) {
+          if (accept) ++inter;
+          else ++opened;
+        }
+        const int nxt = (self || accept) ? (leaf ? cur + 1 : nd.z) : cur + 1;
+        const bool done = nxt <= cur || nxt >= root_skip;  // (never loop on a malformed tree)
+        // The next node's load is in flight while this node's force is
+        // computed (the last step reloads the current node).
+        const BhNode nn = load_node(nodes, done ? cur : nxt);
+        if (accept) bh_force<kFast, kShort>(tm, wp, ax, ay);
+        if (done) break;
+        cur = nxt;
+        nd = nn;
+      }
+      sx = sx + ax;
+      sy = sy + ay;
+    }
+  }
+  if (valid) {
+    fx[p] = sx;
+    fy[p] = sy;
+  }
+  if (kStats && valid) {
+    atomicAdd(&stats[0], inter);
+    atomicAdd(&stats[1], opened);
+  }
+}
+
 // The exact walk takes the short IEEE sequences when this iteration's
 // coordinates and the parameters allow them (a uniform branch).
 template <bool kStats, int W, bool kFast>
@@ -746,10 +833,15 @@ __global__ __launch_bounds__(kBlock) void bh_traverse(
   __shared__ float open_sq[32];
   if (threadIdx.x < 32) open_sq[threadIdx.x] = g->open_sq[threadIdx.x];
   __syncthreads();
-  if (!kFast && wp.short_ok && g->safe_range && g->open_ok)
+  const bool shrt = !kFast && wp.short_ok && g->safe_range && g->open_ok;
+  if (W == 1) {
+    if (shrt) bh_walk_one<kStats, kFast, true>(n, vals, x, y, g, nodes, wp, open_sq, fx, fy, stats);
+    else bh_walk_one<kStats, kFast, false>(n, vals, x, y, g, nodes, wp, open_sq, fx, fy, stats);
+  } else if (shrt) {
     bh_walk_body<kStats, W, kFast, true>(n, vals, x, y, g, nodes, wp, open_sq, fx, fy, stats);
-  else
+  } else {
     bh_walk_body<kStats, W, kFast, false>(n, vals, x, y, g, nodes, wp, open_sq, fx, fy, stats);
+  }
 }
 
 // Exact form of size/dist < theta: theta's rounding boundary (the midpoint
