@@ -553,7 +553,8 @@ __device__ __forceinline__ bool bh_decide(const BhNode& nd, bool leaf, float px,
                                           const float* open_sq, NodeTerm& t) {
   t.dx = px - nd.x;
   t.dy = py - nd.y;
-  t.q = (float)(nd.w & (leaf ? 0x7FFFFFFF : kBhMaxCount));
+  // A leaf's count is below 2^26 too, so one mask serves both encodings.
+  t.q = (float)(nd.w & kBhMaxCount);
   if (kShort) {
     // The exact test as a threshold on the squared distance (walk_open.h);
     // the force takes the square root of accepted cells only. The threshold
@@ -588,9 +589,11 @@ __device__ __forceinline__ void bh_force(const NodeTerm& t, const WalkParams& wp
     ay = fmaf(f, t.dy, ay);
   } else if (kShort) {
     // Below 2^-96 the square root is under 2^-48 and is replaced by min_dist
-    // (>= 2^-20) either way, so 2^-96 may stand in for it.
-    float dist = sqrt_short(fmaxf(t.d, 0x1p-96f));
-    if (dist < wp.min_dist) dist = wp.min_dist;
+    // (>= 2^-20) either way, so 2^-96 may stand in for it. Both maxima are
+    // taken on the bit patterns, which order non-negative floats (d2 is +0
+    // or positive, min_dist positive).
+    float dist = sqrt_short(__int_as_float(max(__float_as_int(t.d), 0x0F800000)));
+    dist = __int_as_float(max(__float_as_int(dist), __float_as_int(wp.min_dist)));
     const float d2 = dist * dist;
     const float f = div_short<false>(wp.k * 1.0f * t.q, d2, recip_short(d2));  // k*q != 0
     const float rd = recip_short(dist);
@@ -626,7 +629,9 @@ __device__ __forceinline__ bool bh_node(const BhNode& nd, bool leaf, float px, f
 // split it into a load of (z, w) and a dependent load of (x, y)).
 __device__ __forceinline__ BhNode load_node(const BhNode* __restrict__ nodes, int c) {
   typedef int v4i __attribute__((ext_vector_type(4)));
-  const v4i v = *(reinterpret_cast<const v4i*>(nodes) + c);
+  // 32-bit byte offset from the base (node counts stay below 2^28).
+  const v4i v = *reinterpret_cast<const v4i*>(reinterpret_cast<const char*>(nodes) +
+                                              ((uint32_t)c << 4));
   BhNode nd;
   nd.x = __int_as_float(v.x);
   nd.y = __int_as_float(v.y);
