@@ -787,7 +787,7 @@ __device__ __forceinline__ void bh_walk_one(
       int qend = node_next(nd, cur);  // end of the current quadrant's subtree
       float ax = 0.0f, ay = 0.0f;
       while (true) {
-        if (cur == qend) {  // the next quadrant's first node
+        if (__builtin_expect(cur == qend, 0)) {  // the next quadrant's first node (rare)
           sx = sx + ax;
           sy = sy + ay;
           ax = 0.0f;
@@ -803,7 +803,9 @@ __device__ __forceinline__ void bh_walk_one(
           if (accept) ++inter;
           else ++opened;
         }
-        const int nxt = (self || accept) ? (leaf ? cur + 1 : nd.z) : cur + 1;
+        // Only an accepted cell is skipped past; every other step goes to the
+        // next node (the first child, or the leaf's successor).
+        const int nxt = (!leaf && accept) ? nd.z : cur + 1;
         const bool done = nxt <= cur || nxt >= root_skip;  // (never loop on a malformed tree)
         // The next node's load is in flight while this node's force is
         // computed (the last step reloads the current node).
