@@ -145,14 +145,19 @@ __global__ __launch_bounds__(kMaxBlock) void place_leaves_kernel(
         } else if (use_short) {
           // The same operations through the short IEEE sequences (positions
           // 0 or of magnitude in [2^-10, 2^20], repel in [2^-20, 2^10]):
-          // below 2^-96 the square root is replaced by 0.1 either way.
+          // below 2^-96 the square root is replaced by 0.1 either way. The
+          // maxima are taken on the bit patterns, which order non-negative
+          // floats. A zero numerator gives +0 where IEEE division gives a
+          // zero of its sign; the sums never tell them apart: they start at
+          // +0, a sum is -0 only when both terms are, and x + (+-0) == x.
           const float dist_sq = dx * dx + dy * dy;
-          const float dist = fmaxf(fx::sqrt_short(fmaxf(dist_sq, 0x1p-96f)), 0.1f);
+          const float root = fx::sqrt_short(__int_as_float(max(__float_as_int(dist_sq), 0x0F800000)));
+          const float dist = __int_as_float(max(__float_as_int(root), 0x3DCCCCCD));  // 0.1f
           const float d2 = dist * dist;
           const float force = fx::div_short<false>(repel, d2, fx::recip_short(d2));
           const float rd = fx::recip_short(dist);
-          const float fx_ij = fx::div_short(force * dx, dist, rd);
-          const float fy_ij = fx::div_short(force * dy, dist, rd);
+          const float fx_ij = fx::div_short<false>(force * dx, dist, rd);
+          const float fy_ij = fx::div_short<false>(force * dy, dist, rd);
           const float nax = ax + fx_ij, nay = ay + fy_ij;
           ax = (j != i) ? nax : ax;
           ay = (j != i) ? nay : ay;
