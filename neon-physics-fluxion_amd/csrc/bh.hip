@@ -255,9 +255,24 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
 // run of equal top halves (two or more keys) orders the run by (key, index)
 // with an insertion sort — linear for runs already in order (equal full keys,
 // e.g. co-located particles).
+// It is also the sort's last kernel (SortFinish): workgroup 0 clears the
+// digit counts and tile counters, and after a timed-out look-back every
+// position gets a zero key and the identity permutation instead.
 __global__ __launch_bounds__(kBlock) void sort_low_halves(int n, uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ vals) {
+                                                          uint32_t* __restrict__ vals,
+                                                          SortFinish fin) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (blockIdx.x == 0) {
+    for (int k = threadIdx.x; k < 8 * 256; k += kBlock) fin.ghist[k] = 0;
+    if (threadIdx.x < 8) fin.counters[threadIdx.x] = 0;
+  }
+  if (*fin.fault) {
+    if (i < n) {
+      keys[i] = 0;
+      vals[i] = (uint32_t)i;
+    }
+    return;
+  }
   if (i + 1 >= n) return;
   const uint32_t top = (uint32_t)(keys[i] >> 32);
   if ((uint32_t)(keys[i + 1] >> 32) != top) return;
@@ -343,42 +358,31 @@ __global__ __launch_bounds__(kBlock) void bh_lcp_count(
   }
 }
 
-// Exclusive prefix of the per-block totals, in place (one workgroup).
-__global__ __launch_bounds__(1024) void tree_scan_sums(TreeSum* __restrict__ tsum, int nb) {
-  __shared__ TreeSum wtot[16];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int per = (nb + 1023) / 1024;
-  const int s0 = t * per, s1 = min(nb, s0 + per);
-  TreeSum acc = {0, 0, 0, 0};
-  for (int k = s0; k < s1; ++k) acc = ts_add(acc, tsum[k]);
-  const TreeSum incl = ts_wave_scan(acc, lane);
-  if (lane == 63) wtot[wave] = incl;
-  __syncthreads();
-  TreeSum run = {0, 0, 0, 0};
-  for (int w = 0; w < wave; ++w) run = ts_add(run, wtot[w]);
-  run.a += incl.a - acc.a;
-  run.b += incl.b - acc.b;
-  run.c += incl.c - acc.c;
-  for (int k = s0; k < s1; ++k) {
-    const TreeSum v = tsum[k];
-    tsum[k] = run;
-    run = ts_add(run, v);
-  }
-}
-
 // pre[i] = sum of tri[0..i), pre[n] = the total: block prefix + in-block scan.
+// Every tile adds up the totals of the tiles before it itself (integer sums,
+// so the order does not matter), which saves a launch for scanning them.
 __global__ __launch_bounds__(kBlock) void tree_scan_tiles(int n, const TreeSum* __restrict__ tri,
                                                           const TreeSum* __restrict__ tsum,
                                                           TreeSum* __restrict__ pre) {
   __shared__ TreeSum wtot[kBlock / 64];
+  __shared__ TreeSum btot[kBlock / 64];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  TreeSum before = {0, 0, 0, 0};
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kBlock) before = ts_add(before, tsum[k]);
+  for (int off = 32; off > 0; off >>= 1) {
+    before.a += __shfl_down(before.a, off, 64);
+    before.b += __shfl_down(before.b, off, 64);
+    before.c += __shfl_down(before.c, off, 64);
+  }
+  if (lane == 0) btot[wave] = before;
   TreeSum v = {0, 0, 0, 0};
   if (i < n) v = tri[i];
   const TreeSum incl = ts_wave_scan(v, lane);
   if (lane == 63) wtot[wave] = incl;
   __syncthreads();
-  TreeSum run = tsum[blockIdx.x];
+  TreeSum run = btot[0];
+  for (int w = 1; w < kBlock / 64; ++w) run = ts_add(run, btot[w]);
   for (int w = 0; w < wave; ++w) run = ts_add(run, wtot[w]);
   TreeSum ex = run;
   ex.a += incl.a - v.a;
@@ -987,11 +991,15 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
   // of equal top halves — particles closer than the 2^-16 root fraction, rare
   // — ordered by the full key in place: the result is the stable sort by the
   // full key, as the indices enter in ascending order.
-  if ((rc = radix_sort_pairs(j.ctx, st, b.sort, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), n, 32, 64)))
-    return rc;
-  hipLaunchKernelGGL(sort_low_halves, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
-                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>());
-  FX_HIP(hipGetLastError());
+  if (n >= 2) {  // (one key is sorted; the sort then sets nothing up)
+    SortFinish fin;
+    if ((rc = radix_sort_pairs(j.ctx, st, b.sort, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(),
+                               n, 32, 64, &fin)))
+      return rc;
+    hipLaunchKernelGGL(sort_low_halves, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
+                       b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), fin);
+    FX_HIP(hipGetLastError());
+  }
   if ((rc = phase_end(j, kPhaseBhSort))) return rc;
   if (after_sort) {
     b.sorted_valid = true;
@@ -1003,7 +1011,6 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
                      (const BhGeometry*)g, b.lcp.as<int8_t>(), b.m1.as<int8_t>(), b.m2.as<int8_t>(),
                      b.mhi.as<int>(), (int)b.hi_stride, b.tri.as<TreeSum>(), b.tsum.as<TreeSum>());
   FX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(tree_scan_sums, dim3(1), dim3(1024), 0, st, b.tsum.as<TreeSum>(), (int)b.nb1);
   hipLaunchKernelGGL(tree_scan_tiles, dim3((unsigned)b.nb1), dim3(kBlock), 0, st, n,
                      (const TreeSum*)b.tri.as<TreeSum>(), (const TreeSum*)b.tsum.as<TreeSum>(),
                      b.pre.as<TreeSum>());

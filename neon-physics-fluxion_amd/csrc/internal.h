@@ -72,10 +72,21 @@ struct SortWorkspace {
   // Allocates for n items and clears the look-back state on `st`.
   int ensure(fx_ctx* ctx, hipStream_t st, int64_t n);
 };
+// The last kernel of a sort clears the digit counts (ghist, 8 x 256) and the
+// tile counters (8) for the next sort, and after a timed-out look-back
+// (*fault != 0) replaces the output by zero keys and the identity
+// permutation. A caller that passes `finish` gets these pointers and does
+// that in its own next kernel instead (one launch fewer).
+struct SortFinish {
+  uint32_t* ghist;
+  uint32_t* counters;
+  const uint32_t* fault;
+};
 // Stable LSD radix sort over key bits [lo_bit, hi_bit). keys/vals are sorted
 // in place (ping-pong through the workspace). Enqueued on `stream`.
 int radix_sort_pairs(fx_ctx* ctx, hipStream_t stream, SortWorkspace& ws, uint64_t* keys,
-                     uint32_t* vals, int64_t n, int lo_bit, int hi_bit);
+                     uint32_t* vals, int64_t n, int lo_bit, int hi_bit,
+                     SortFinish* finish = nullptr);
 // Whether a look-back wait of any sort on this workspace timed out (host sync).
 int radix_fault(SortWorkspace& ws, hipStream_t stream, bool* faulted);
 

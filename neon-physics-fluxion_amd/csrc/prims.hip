@@ -209,7 +209,7 @@ int SortWorkspace::ensure(fx_ctx* ctx, hipStream_t st, int64_t n) {
 }
 
 int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* keys,
-                     uint32_t* vals, int64_t n, int lo_bit, int hi_bit) {
+                     uint32_t* vals, int64_t n, int lo_bit, int hi_bit, SortFinish* finish) {
   if (n <= 1) return FX_OK;
   int rc = ws.ensure(ctx, st, n);
   if (rc) return rc;
@@ -246,6 +246,12 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
       return rc;
     FX_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
     FX_HIP(hipMemcpyAsync(vals, va, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  }
+  if (finish) {
+    finish->ghist = gbase;
+    finish->counters = counters;
+    finish->fault = fault;
+    return FX_OK;
   }
   hipLaunchKernelGGL(radix_repair, dim3((unsigned)std::min<int64_t>(div_up(n, 256), 1024)),
                      dim3(256), 0, st, (const uint32_t*)fault, gbase, counters, keys, vals, n);
