@@ -786,11 +786,14 @@ __device__ __forceinline__ void bh_walk_one(
       }
     }
     if (split && root_skip > 1) {
-      int cur = 1;
-      BhNode nd = load_node(nodes, cur);
-      int qend = node_next(nd, cur);  // end of the current quadrant's subtree
+      BhNode na = load_node(nodes, 1), nb;
+      int ca = 1, cb = 0;
+      int qend = node_next(na, 1);  // end of the current quadrant's subtree
       float ax = 0.0f, ay = 0.0f;
-      while (true) {
+      // One node visit; true when the walk is done, else (nxt, nn) is the
+      // next node. Two visits per loop trip alternate the registers of the
+      // current and the next node.
+      auto visit = [&](const BhNode& nd, int cur, int& nxt, BhNode& nn) -> bool {
         if (__builtin_expect(cur == qend, 0)) {  // the next quadrant's first node (rare)
           sx = sx + ax;
           sy = sy + ay;
@@ -809,15 +812,15 @@ __device__ __forceinline__ void bh_walk_one(
         }
         // Only an accepted cell is skipped past; every other step goes to the
         // next node (the first child, or the leaf's successor).
-        const int nxt = (!leaf && accept) ? nd.z : cur + 1;
+        nxt = (!leaf && accept) ? nd.z : cur + 1;
         const bool done = nxt <= cur || nxt >= root_skip;  // (never loop on a malformed tree)
         // The next node's load is in flight while this node's force is
         // computed (the last step reloads the current node).
-        const BhNode nn = load_node(nodes, done ? cur : nxt);
+        nn = load_node(nodes, done ? cur : nxt);
         if (accept) bh_force<kFast, kShort>(tm, wp, ax, ay);
-        if (done) break;
-        cur = nxt;
-        nd = nn;
+        return done;
+      };
+      while (!visit(na, ca, cb, nb) && !visit(nb, cb, ca, na)) {
       }
       sx = sx + ax;
       sy = sy + ay;
