@@ -382,6 +382,11 @@ int fx_circuit_netlist(const fx_circuit* c, int32_t* net_ptr, int32_t* net_pins,
 const char* fx_circuit_name(const fx_circuit* c);
 const char* fx_circuit_net_name(const fx_circuit* c, int32_t net);
 const char* fx_circuit_gate_name(const fx_circuit* c, int32_t gate);
+/* Every net name (Net.name, HAPR_fluxion_v4/src/data/circuit.rs), each
+ * NUL-terminated, back to back in net order: *needed = the bytes required;
+ * with buf NULL only *needed is set; FX_E_INVALID if buf_len < *needed. One
+ * call in place of num_nets calls of fx_circuit_net_name. */
+int fx_circuit_net_names(const fx_circuit* c, char* buf, int64_t buf_len, int64_t* needed);
 /* def::write_def (HAPR_fluxion_v4/src/parser/def.rs:12-41), byte for byte:
  * coordinates are the FP32 products x*1000 cast as Rust's `as i64` does
  * (truncation, NaN -> 0, saturation). gate_names NULL: g<i>. */

@@ -21,10 +21,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
-#include <fstream>
 #include <limits>
 #include <string>
-#include <unordered_map>
+#include <string_view>
 #include <vector>
 
 #include "fluxion_hip.h"
@@ -51,18 +50,65 @@ int set_err(int code, const std::string& msg) { return fx::fail(code, "%s", msg.
 
 bool is_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\f' || c == '\v'; }
 
-std::vector<std::string> split_ws(const std::string& s) {
-  std::vector<std::string> out;
+// Whitespace-separated tokens of one line, as views into the file buffer.
+void split_ws(std::string_view s, std::vector<std::string_view>& out) {
+  out.clear();
   size_t i = 0;
   while (i < s.size()) {
     while (i < s.size() && is_space(s[i])) ++i;
     size_t j = i;
     while (j < s.size() && !is_space(s[j])) ++j;
-    if (j > i) out.emplace_back(s, i, j - i);
+    if (j > i) out.push_back(s.substr(i, j - i));
     i = j;
   }
-  return out;
 }
+
+// Net names -> ids in first-seen order: open addressing with linear probing
+// over a power-of-two table of ids; names are views into the file buffer.
+class NameTable {
+ public:
+  // The id of `s`, adding it as names.size() when new.
+  int32_t find_or_add(std::string_view s, std::vector<std::string_view>& names, bool* added) {
+    if (2 * (names.size() + 1) > slot_.size()) grow(names);
+    const uint64_t h = hash(s);
+    for (size_t i = h & mask_;; i = (i + 1) & mask_) {
+      const int32_t id = slot_[i];
+      if (id < 0) {
+        slot_[i] = (int32_t)names.size();
+        hashes_.push_back(h);
+        names.push_back(s);
+        *added = true;
+        return slot_[i];
+      }
+      if (hashes_[id] == h && names[id] == s) {
+        *added = false;
+        return id;
+      }
+    }
+  }
+
+ private:
+  static uint64_t hash(std::string_view s) {  // FNV-1a, then a final mix
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char ch : s) h = (h ^ ch) * 1099511628211ull;
+    h ^= h >> 29;
+    h *= 0xbf58476d1ce4e5b9ull;
+    return h ^ (h >> 32);
+  }
+  void grow(const std::vector<std::string_view>& names) {
+    const size_t cap = slot_.empty() ? 1024 : 2 * slot_.size();
+    slot_.assign(cap, -1);
+    mask_ = cap - 1;
+    for (int32_t id = 0; id < (int32_t)names.size(); ++id) {
+      size_t i = hashes_[id] & mask_;
+      while (slot_[i] >= 0) i = (i + 1) & mask_;
+      slot_[i] = id;
+    }
+  }
+  std::vector<int32_t> slot_;
+  std::vector<uint64_t> hashes_;
+  size_t mask_ = 0;
+};
 
 // Path::file_stem: the file name without its last extension; a name that
 // starts with its only dot is kept whole.
@@ -81,47 +127,61 @@ extern "C" int fx_blif_read(const char* path, fx_circuit** out) {
   fx::clear_error();
   if (!path || !out) return set_err(FX_E_INVALID, "fx_blif_read: bad arguments");
   *out = nullptr;
-  std::ifstream in(path, std::ios::binary);
-  if (!in) return set_err(FX_E_INVALID, std::string("Failed to open BLIF: ") + path);
+  // The whole file in memory; tokens and net names are views into it.
+  std::string text;
+  {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_err(FX_E_INVALID, std::string("Failed to open BLIF: ") + path);
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
+    const bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (err) return set_err(FX_E_INVALID, std::string("error reading BLIF: ") + path);
+  }
   auto* c = new fx_circuit();
   c->name = file_stem(path);
   // net name -> index into `nets` (first-seen order); driver per net
-  std::unordered_map<std::string, int32_t> net_id;
-  std::vector<std::string> nets;
+  NameTable net_id;
+  std::vector<std::string_view> nets;
   std::vector<int32_t> driver;     // -1: not driven
   std::vector<int32_t> first_drv;  // sequence number of the first drive, -1 if never
   std::vector<int32_t> first_read;
-  std::vector<std::vector<int32_t>> readers;
+  std::vector<int32_t> read_net, read_gate;  // reads in file order
   int32_t drive_seq = 0, read_seq = 0;
-  auto id_of = [&](const std::string& s) {
-    auto it = net_id.find(s);
-    if (it != net_id.end()) return it->second;
-    const int32_t id = (int32_t)nets.size();
-    net_id.emplace(s, id);
-    nets.push_back(s);
-    driver.push_back(-1);
-    first_drv.push_back(-1);
-    first_read.push_back(-1);
-    readers.emplace_back();
+  auto id_of = [&](std::string_view s) {
+    bool added = false;
+    const int32_t id = net_id.find_or_add(s, nets, &added);
+    if (added) {
+      driver.push_back(-1);
+      first_drv.push_back(-1);
+      first_read.push_back(-1);
+    }
     return id;
   };
-  auto drive = [&](const std::string& s, int32_t g) {
+  auto drive = [&](std::string_view s, int32_t g) {
     const int32_t id = id_of(s);
     driver[id] = g;
     if (first_drv[id] < 0) first_drv[id] = drive_seq++;
   };
-  auto read = [&](const std::string& s, int32_t g) {
+  auto read = [&](std::string_view s, int32_t g) {
     const int32_t id = id_of(s);
-    readers[id].push_back(g);
+    read_net.push_back(id);
+    read_gate.push_back(g);
     if (first_read[id] < 0) first_read[id] = read_seq++;
   };
-  std::string line;
+  std::vector<std::string_view> tok;
   int64_t lineno = 0;
-  while (std::getline(in, line)) {
+  const std::string_view all(text);
+  for (size_t pos = 0; pos < all.size();) {
+    size_t eol = all.find('\n', pos);
+    if (eol == std::string_view::npos) eol = all.size();
+    const std::string_view line = all.substr(pos, eol - pos);
+    pos = eol + 1;
     ++lineno;
-    const std::vector<std::string> tok = split_ws(line);
+    split_ws(line, tok);
     if (tok.empty() || tok[0][0] == '#') continue;
-    const std::string& d = tok[0];
+    const std::string_view d = tok[0];
     if (d != ".names" && d != ".latch" && d != ".gate") continue;
     const int32_t g = (int32_t)c->gate_names.size();
     if (g == std::numeric_limits<int32_t>::max()) {
@@ -150,33 +210,40 @@ extern "C" int fx_blif_read(const char* path, fx_circuit** out) {
                                          ": .gate without a gate type");
       }
       for (size_t k = 2; k < tok.size(); ++k) {
-        const std::string& t = tok[k];
+        const std::string_view t = tok[k];
         const size_t eq = t.find('=');
-        if (eq == std::string::npos) continue;
-        const std::string net = t.substr(eq + 1);
-        if (t.compare(0, 2, "O=") == 0 || t.compare(0, 2, "Y=") == 0)
+        if (eq == std::string_view::npos) continue;
+        const std::string_view net = t.substr(eq + 1);
+        if (t.substr(0, 2) == "O=" || t.substr(0, 2) == "Y=")
           drive(net, g);
         else
           read(net, g);
       }
     }
   }
-  if (in.bad()) {
-    delete c;
-    return set_err(FX_E_INVALID, std::string("error reading BLIF: ") + path);
+  // Readers of each net in file order: a counting sort of the reads by net.
+  const int32_t nn = (int32_t)nets.size();
+  std::vector<int64_t> rptr(nn + 1, 0);
+  for (int32_t id : read_net) ++rptr[id + 1];
+  for (int32_t id = 0; id < nn; ++id) rptr[id + 1] += rptr[id];
+  std::vector<int32_t> readers(read_net.size());
+  {
+    std::vector<int64_t> cur(rptr.begin(), rptr.end() - 1);
+    for (size_t k = 0; k < read_net.size(); ++k) readers[cur[read_net[k]]++] = read_gate[k];
   }
   // Driven nets by first drive, then readers-only nets by first read.
   std::vector<int32_t> seq_drv(drive_seq, -1), seq_read(read_seq, -1);
-  for (int32_t id = 0; id < (int32_t)nets.size(); ++id) {
+  for (int32_t id = 0; id < nn; ++id) {
     if (first_drv[id] >= 0) seq_drv[first_drv[id]] = id;
     else if (first_read[id] >= 0) seq_read[first_read[id]] = id;
   }
   auto add_net = [&](int32_t id, bool with_driver) {
-    const size_t k = readers[id].size() + (with_driver ? 1 : 0);
+    const int64_t nr = rptr[id + 1] - rptr[id];
+    const int64_t k = nr + (with_driver ? 1 : 0);
     if (k < 2) return;
-    c->net_names.push_back(nets[id]);
+    c->net_names.emplace_back(nets[id]);
     if (with_driver) c->net_pins.push_back(driver[id]);
-    c->net_pins.insert(c->net_pins.end(), readers[id].begin(), readers[id].end());
+    c->net_pins.insert(c->net_pins.end(), readers.begin() + rptr[id], readers.begin() + rptr[id + 1]);
     c->net_ptr.push_back((int32_t)c->net_pins.size());
   };
   for (int32_t id : seq_drv)
@@ -220,6 +287,22 @@ extern "C" const char* fx_circuit_name(const fx_circuit* c) { return c ? c->name
 extern "C" const char* fx_circuit_net_name(const fx_circuit* c, int32_t net) {
   if (!c || net < 0 || net >= (int32_t)c->net_names.size()) return nullptr;
   return c->net_names[net].c_str();
+}
+
+extern "C" int fx_circuit_net_names(const fx_circuit* c, char* buf, int64_t buf_len,
+                                    int64_t* needed) {
+  fx::clear_error();
+  if (!c || !needed) return set_err(FX_E_INVALID, "fx_circuit_net_names: bad arguments");
+  int64_t total = 0;
+  for (const std::string& n : c->net_names) total += (int64_t)n.size() + 1;
+  *needed = total;
+  if (!buf) return FX_OK;
+  if (buf_len < total) return set_err(FX_E_INVALID, "fx_circuit_net_names: buffer too small");
+  for (const std::string& n : c->net_names) {
+    std::memcpy(buf, n.c_str(), n.size() + 1);
+    buf += n.size() + 1;
+  }
+  return FX_OK;
 }
 
 extern "C" const char* fx_circuit_gate_name(const fx_circuit* c, int32_t gate) {

@@ -189,6 +189,7 @@ _SIGS = {
     "fx_circuit_name": (C.c_char_p, [_P]),
     "fx_circuit_net_name": (C.c_char_p, [_P, _I32]),
     "fx_circuit_gate_name": (C.c_char_p, [_P, _I32]),
+    "fx_circuit_net_names": (_I32, [_P, _P, _I64, _P]),
     "fx_def_write": (C.c_int, [C.c_char_p, C.c_char_p, _I32, _P, _P, _P, _F, _F]),
     "fx_build_adjacency": (_I64, [_I32, _I32, _P, _P, _P, _P, _I64]),
     "fx_fiedler_init": (C.c_int, [_I32, _P]),

@@ -32,7 +32,11 @@ def parse_blif(path) -> Circuit:
         net_pins = np.zeros(max(npins.value, 1), np.int32)
         types = np.zeros(max(ng.value, 1), np.uint8)
         check(lib.fx_circuit_netlist(h, ptr(net_ptr), ptr(net_pins), ptr(types)))
-        names = [lib.fx_circuit_net_name(h, e).decode() for e in range(ne.value)]
+        need = C.c_int64()
+        check(lib.fx_circuit_net_names(h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(max(need.value, 1))
+        check(lib.fx_circuit_net_names(h, buf, need.value, C.byref(need)))
+        names = buf.raw[:need.value].decode().split("\0")[:-1] if need.value else []
         name = lib.fx_circuit_name(h).decode()
     finally:
         lib.fx_circuit_free(h)
