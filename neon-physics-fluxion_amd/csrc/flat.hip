@@ -92,11 +92,24 @@ __global__ void combine_kernel(CombineArgs a) {
   const float xi = a.x[i], yi = a.y[i];
   float sx = 0.0f, sy = 0.0f;
   if (a.comps & 1) {
+    // Nets in chunks of kChunk: all their indices, then all their centroids
+    // are loaded before the terms are added in net order (independent loads
+    // in flight together instead of two dependent loads per net).
+    constexpr int kChunk = 8;
     const int q1 = a.gate_ptr[i + 1];
-    for (int q = a.gate_ptr[i]; q < q1; ++q) {
-      const float4 c = a.netc[a.gate_net[q]];
-      sx += c.z * (c.x - xi);
-      sy += c.z * (c.y - yi);
+    for (int q0 = a.gate_ptr[i]; q0 < q1; q0 += kChunk) {
+      int e[kChunk];
+      float4 c[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) e[k] = q0 + k < q1 ? a.gate_net[q0 + k] : -1;
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) if (e[k] >= 0) c[k] = a.netc[e[k]];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        if (e[k] < 0) break;
+        sx += c[k].z * (c[k].x - xi);
+        sy += c[k].z * (c[k].y - yi);
+      }
     }
   }
   float bx = 0.0f, by = 0.0f;
