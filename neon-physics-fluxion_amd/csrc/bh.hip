@@ -35,6 +35,7 @@
 #include <cstring>
 
 #include "job.h"
+#include "run_sort.h"
 #include "walk_open.h"
 
 namespace fx {
@@ -253,8 +254,9 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
 
 // After a stable sort on the top 32 key bits: the thread at the head of each
 // run of equal top halves (two or more keys) orders the run by (key, index)
-// with an insertion sort — linear for runs already in order (equal full keys,
-// e.g. co-located particles).
+// (run_sort.h: insertion sort up to 32 keys — linear for runs already in
+// order, e.g. co-located particles — a merge sort through the sort's free
+// scratch buffers beyond).
 // It is also the sort's last kernel (SortFinish): workgroup 0 clears the
 // digit counts and tile counters, and after a timed-out look-back every
 // position gets a zero key and the identity permutation instead.
@@ -279,18 +281,7 @@ __global__ __launch_bounds__(kBlock) void sort_low_halves(int n, uint64_t* __res
   if (i > 0 && (uint32_t)(keys[i - 1] >> 32) == top) return;
   int e = i + 2;
   while (e < n && (uint32_t)(keys[e] >> 32) == top) ++e;
-  for (int a = i + 1; a < e; ++a) {
-    const uint64_t k = keys[a];
-    const uint32_t v = vals[a];
-    int b = a - 1;
-    while (b >= i && (keys[b] > k || (keys[b] == k && vals[b] > v))) {
-      keys[b + 1] = keys[b];
-      vals[b + 1] = vals[b];
-      --b;
-    }
-    keys[b + 1] = k;
-    vals[b + 1] = v;
-  }
+  sort_run(keys + i, vals + i, fin.keys_scratch + i, fin.vals_scratch + i, e - i);
 }
 
 __device__ __forceinline__ int lcp_levels(const uint64_t* __restrict__ keys, int n, int a, int D) {

@@ -81,6 +81,8 @@ struct SortFinish {
   uint32_t* ghist;
   uint32_t* counters;
   const uint32_t* fault;
+  uint64_t* keys_scratch;  // the workspace's n-entry buffers, free once the sort is done
+  uint32_t* vals_scratch;
 };
 // Stable LSD radix sort over key bits [lo_bit, hi_bit). keys/vals are sorted
 // in place (ping-pong through the workspace). Enqueued on `stream`.

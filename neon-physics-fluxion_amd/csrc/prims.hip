@@ -251,6 +251,8 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
     finish->ghist = gbase;
     finish->counters = counters;
     finish->fault = fault;
+    finish->keys_scratch = ws.keys_alt.as<uint64_t>();
+    finish->vals_scratch = ws.vals_alt.as<uint32_t>();
     return FX_OK;
   }
   hipLaunchKernelGGL(radix_repair, dim3((unsigned)std::min<int64_t>(div_up(n, 256), 1024)),

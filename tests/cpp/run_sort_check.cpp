@@ -1,0 +1,52 @@
+// CPU check of the Barnes-Hut key sort's run ordering (csrc/run_sort.h)
+// against std::sort by (key, value): runs of every length up to 300 and some
+// long ones, with many equal keys, already-sorted and reversed inputs.
+// Exits 1 on any difference.
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <random>
+#include <utility>
+#include <vector>
+
+#include "run_sort.h"
+
+int main() {
+  std::mt19937_64 g(11);
+  long long runs = 0, bad = 0;
+  auto check = [&](std::vector<uint64_t> k, std::vector<uint32_t> v) {
+    const int n = (int)k.size();
+    std::vector<std::pair<uint64_t, uint32_t>> want(n);
+    for (int i = 0; i < n; ++i) want[i] = {k[i], v[i]};
+    std::sort(want.begin(), want.end());
+    std::vector<uint64_t> ks(n + 1);
+    std::vector<uint32_t> vs(n + 1);
+    fx::sort_run(k.data(), v.data(), ks.data(), vs.data(), n);
+    ++runs;
+    for (int i = 0; i < n; ++i)
+      if (k[i] != want[i].first || v[i] != want[i].second) {
+        if (bad < 5) printf("mismatch n=%d at %d\n", n, i);
+        ++bad;
+        return;
+      }
+  };
+  std::vector<int> lens;
+  for (int n = 0; n <= 300; ++n) lens.push_back(n);
+  for (int n : {511, 512, 513, 1000, 4096, 20000}) lens.push_back(n);
+  for (int n : lens) {
+    for (int mode = 0; mode < 4; ++mode) {
+      std::vector<uint64_t> k(n);
+      std::vector<uint32_t> v(n);
+      const uint64_t top = 0xABCD123400000000ull;
+      for (int i = 0; i < n; ++i) {
+        k[i] = top | (mode == 1 ? g() % 4 : g() % 0xFFFFFFFFull);  // mode 1: many equal keys
+        v[i] = (uint32_t)i * 7919u % 1000003u;                     // distinct values
+      }
+      if (mode == 2) std::sort(k.begin(), k.end());
+      if (mode == 3) std::sort(k.rbegin(), k.rend());
+      check(k, v);
+    }
+  }
+  printf("runs %lld mismatches %lld\n", runs, bad);
+  return bad ? 1 : 0;
+}

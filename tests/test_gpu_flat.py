@@ -137,9 +137,9 @@ def test_barnes_hut_colocated_and_clamped():
 
 
 def test_barnes_hut_clustered():
-    # Dense clusters: the key sort's buckets (cells of the first 8 tree levels)
-    # then hold tens of thousands of keys (merge path), hundreds (LDS path)
-    # and a few (wave path); the walk stays bit-identical to the path of record.
+    # Dense clusters (50,000 particles within ~1 unit, 100 at one point): deep
+    # trees, merged leaves and runs of equal top key halves; the walk stays
+    # bit-identical to the path of record.
     print("case: BH clustered", flush=True)
     rng = np.random.default_rng(12)
     N = 80000
@@ -151,6 +151,30 @@ def test_barnes_hut_clustered():
     y[50000:56000] = rng.normal(900.0, 6.0, 6000).astype(np.float32)
     x[56000:56100] = 250.0
     y[56000:56100] = 250.0
+    circ = fxn.Circuit.from_nets(N, [[i, (i + 1) % N] for i in range(N)])
+    job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig())
+    job.set_positions(x, y)
+    gx, gy = job.forces(2)
+    rx, ry, st = orc.bh_record(x, y, nthreads=8)
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    s = job.bh_stats()
+    assert s["nodes"] == st["nodes"] and s["interactions"] == st["interactions"]
+
+
+def test_barnes_hut_long_runs():
+    # 6,000 particles packed into a square of 0.004 inside a die of 1000: the
+    # key sort's runs of equal top halves (cells of 2^-16 of the root) hold
+    # thousands of keys, ordered by the fix-up's merge sort (runs of up to 32
+    # take its insertion sort), with many co-located duplicates among them.
+    print("case: BH long runs", flush=True)
+    rng = np.random.default_rng(31)
+    N = 6000
+    x = (500.0 + rng.uniform(0.0, 0.004, N)).astype(np.float32)
+    y = (500.0 + rng.uniform(0.0, 0.004, N)).astype(np.float32)
+    x[:40] = (250.0 + rng.uniform(0.0, 0.004, 40)).astype(np.float32)
+    y[:40] = 250.0
+    x[40:42] = [0.0, 1000.0]
+    y[40:42] = [0.0, 1000.0]
     circ = fxn.Circuit.from_nets(N, [[i, (i + 1) % N] for i in range(N)])
     job = fxn.FlatJob(fxn_ctx[0], circ, fxn.PlacerConfig())
     job.set_positions(x, y)

@@ -1,0 +1,24 @@
+"""The Barnes-Hut key sort's ordering of runs of equal top halves (csrc/run_sort.h:
+insertion sort for short runs, merge sort through scratch for long ones) against
+std::sort by (key, particle index), compiled and run on the CPU
+(tests/cpp/run_sort_check.cpp)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_run_sort_matches_std_sort(tmp_path):
+    exe = str(tmp_path / "run_sort_check")
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "-x", "hip", "--offload-arch=gfx950",
+                    "-I", os.path.join(ROOT, "neon-physics-fluxion_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "cpp", "run_sort_check.cpp"), "-o", exe],
+                   check=True, capture_output=True, timeout=300)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " mismatches 0" in r.stdout
