@@ -20,6 +20,14 @@ import os
 import sys
 import time
 
+# The serving leg runs up to 16 placement jobs, each on a stream of its own.
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set) and
+# streams sharing a queue run one after another, so one queue per active job
+# (measured: burst 38.9 -> 44.3 placements/s). Read when HIP initialises,
+# i.e. before the first GPU call of this process.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or "4") < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "neon-physics-fluxion_amd"))
 sys.path.insert(0, ROOT)
