@@ -757,7 +757,12 @@ extern "C" void fx_job_destroy(fx_job* job) {
   fx::drop_timing(j);
   hipStream_t s = j.stream, a = j.aux;
   hipEvent_t e0 = j.ev_fork, e1 = j.ev_join, e2 = j.ev_start;
-  delete job;  // DevBufs release their memory
+  {
+    // The job's streams are idle: its buffers go to the context's pool
+    // (hipFree would wait for every job on the device).
+    fx::RecycleScope keep(j.ctx);
+    delete job;
+  }
   if (e2) (void)hipEventDestroy(e2);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

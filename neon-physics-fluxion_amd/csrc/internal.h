@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -25,6 +26,12 @@ struct fx_ctx {
   double hierarchy_ms = 0.0;
   double attention_ms = 0.0;
   std::mutex mu;  // serialises calls that use the context's own stream
+  // Idle device blocks by size, kept for reuse: hipFree waits for the whole
+  // device, so buffers of destroyed jobs (their streams synchronised) come
+  // here instead and serve later allocations; freed at fx_ctx_destroy.
+  std::mutex pool_mu;
+  std::multimap<size_t, void*> pool;
+  int64_t pool_bytes = 0;
 };
 
 namespace fx {
@@ -47,6 +54,17 @@ void clear_error();
   } while (0)
 
 // Device buffer owned by a context; counts bytes against the context.
+// While a RecycleScope for a context is open on this thread, DevBufs of that
+// context released on it go to the context's pool instead of hipFree. Open
+// one only where no queued GPU work can still use the buffers released in it.
+struct RecycleScope {
+  fx_ctx* prev;
+  explicit RecycleScope(fx_ctx* c);
+  ~RecycleScope();
+};
+// Returns the pooled blocks of a context to HIP.
+void pool_clear(fx_ctx* c);
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;

@@ -3,6 +3,7 @@
 // fx_ctx_create: it binds one HIP device and one stream of default priority;
 // a missing device is an error returned to the caller (no CPU fallback).
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 
@@ -22,11 +23,49 @@ int fail(int code, const char* fmt, ...) {
 
 void clear_error() { g_err[0] = 0; }
 
+static thread_local fx_ctx* g_recycle = nullptr;
+// Largest total the pool of one context keeps (beyond it, blocks are freed).
+constexpr int64_t kPoolMaxBytes = int64_t(32) << 30;
+
+RecycleScope::RecycleScope(fx_ctx* c) : prev(g_recycle) { g_recycle = c; }
+RecycleScope::~RecycleScope() { g_recycle = prev; }
+
+// The smallest pooled block of n bytes or more, if it is not much larger.
+static bool pool_take(fx_ctx* c, size_t n, void** p, size_t* bytes) {
+  std::lock_guard<std::mutex> lk(c->pool_mu);
+  auto it = c->pool.lower_bound(n);
+  if (it == c->pool.end() || it->first > std::max(2 * n, n + (size_t(1) << 20))) return false;
+  *p = it->second;
+  *bytes = it->first;
+  c->pool_bytes -= (int64_t)it->first;
+  c->pool.erase(it);
+  return true;
+}
+
+static bool pool_put(fx_ctx* c, void* p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(c->pool_mu);
+  if (c->pool_bytes + (int64_t)bytes > kPoolMaxBytes) return false;
+  c->pool.emplace(bytes, p);
+  c->pool_bytes += (int64_t)bytes;
+  return true;
+}
+
+void pool_clear(fx_ctx* c) {
+  std::lock_guard<std::mutex> lk(c->pool_mu);
+  for (auto& kv : c->pool) (void)hipFree(kv.second);
+  c->pool.clear();
+  c->pool_bytes = 0;
+}
+
 int DevBuf::alloc(fx_ctx* c, size_t n) {
   if (p && bytes >= n) return FX_OK;
   release();
   ctx = c;
   if (n == 0) n = 256;
+  if (c && pool_take(c, n, &p, &bytes)) {
+    c->bytes += (int64_t)bytes;
+    return FX_OK;
+  }
   hipError_t e = hipMalloc(&p, n);
   if (e != hipSuccess) {
     p = nullptr;
@@ -40,8 +79,8 @@ int DevBuf::alloc(fx_ctx* c, size_t n) {
 
 void DevBuf::release() {
   if (p) {
-    (void)hipFree(p);
     if (ctx) ctx->bytes -= (int64_t)bytes;
+    if (!(ctx && g_recycle == ctx && pool_put(ctx, p, bytes))) (void)hipFree(p);
   }
   p = nullptr;
   bytes = 0;
@@ -107,6 +146,7 @@ extern "C" void fx_ctx_destroy(fx_ctx* ctx) {
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  fx::pool_clear(ctx);
   delete ctx;
 }
 
