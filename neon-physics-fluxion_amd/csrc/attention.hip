@@ -151,15 +151,33 @@ __global__ void run_counts(int32_t R, int64_t Z, const int64_t* __restrict__ hea
                            int32_t* __restrict__ shared, int32_t* __restrict__ total,
                            int32_t* __restrict__ offdiag, int32_t* __restrict__ row_cnt) {
   const int32_t u = blockIdx.x * kB + threadIdx.x;
-  if (u >= R) return;
-  const int64_t end = u + 1 < R ? head_at[u + 1] : Z;
-  const int32_t c = (int32_t)(end - head_at[u]);
-  shared[u] = c;
-  const uint64_t mask = (1ull << cbits) - 1;
-  const int32_t i = (int32_t)(ukey[u] >> cbits), j = (int32_t)(ukey[u] & mask);
-  if (i == j) total[i] = c;
-  offdiag[u] = i != j;
-  if (i != j) atomicAdd(&row_cnt[i], 1);
+  const bool valid = u < R;
+  int32_t i = -1, j = -1;
+  if (valid) {
+    const int64_t end = u + 1 < R ? head_at[u + 1] : Z;
+    const int32_t c = (int32_t)(end - head_at[u]);
+    shared[u] = c;
+    const uint64_t mask = (1ull << cbits) - 1;
+    i = (int32_t)(ukey[u] >> cbits);
+    j = (int32_t)(ukey[u] & mask);
+    if (i == j) total[i] = c;
+    offdiag[u] = i != j;
+  }
+  // Pairs come sorted by row, so a row's pairs are consecutive: the first
+  // lane of each row's stretch in the wave adds the stretch's off-diagonal
+  // count with one atomic (rows of thousands of pairs contended before).
+  const int lane = threadIdx.x & 63;
+  const int32_t prev_i = __shfl_up(i, 1, 64);
+  const bool head = valid && (lane == 0 || prev_i != i);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long od = __ballot(valid && i != j);
+  if (head) {
+    const unsigned long long above = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    const unsigned long long upto = next == 64 ? ~0ull : ((1ull << next) - 1);
+    const int cnt = __popcll(od & upto & (~0ull << lane));
+    if (cnt) atomicAdd(&row_cnt[i], cnt);
+  }
 }
 
 // Off-diagonal entries in (i, j) order with their FP32 scores, and the
