@@ -325,7 +325,9 @@ int32_t fx_attention_num_levels(const fx_attention* a);
 int fx_attention_level_info(const fx_attention* a, int32_t level, int32_t* level_index,
                             int32_t* num_clusters, int64_t* nnz);
 /* cluster_node[n]: hierarchy node of each cluster; total[n]; row_ptr[n+1];
- * col/shared/score[nnz]; top[n*top_k] (-1 padded). Any pointer may be NULL. */
+ * col/shared/score[nnz]; top[n*top_k] (-1 padded). Any pointer may be NULL.
+ * col, shared, score and top stay in device memory until this call copies
+ * them out (fx_attention_free releases them). */
 int fx_attention_level_get(const fx_attention* a, int32_t level, int32_t* cluster_node,
                            int32_t* total, int64_t* row_ptr, int32_t* col, int32_t* shared,
                            float* score, int32_t* top);

@@ -20,6 +20,7 @@
 // division per entry.
 
 #include <algorithm>
+#include <deque>
 #include <memory>
 #include <vector>
 
@@ -31,12 +32,14 @@ struct fx_attention {
     std::vector<int32_t> nodes;    // cluster -> hierarchy node id
     std::vector<int32_t> total;    // |N_i|
     std::vector<int64_t> row_ptr;  // CSR of the non-zero off-diagonal scores
-    std::vector<int32_t> col, shared;
-    std::vector<float> score;
-    std::vector<int32_t> top;      // [n * k], -1 padded
+    int64_t nnz = 0;
+    // The CSR columns, shared counts and scores ([nnz]) and the top-k lists
+    // ([n * k], -1 padded) stay in device memory until fx_attention_level_get
+    // copies them out (at 2M gates they hold ~125 M entries over all levels).
+    fx::DevBuf col, shared, score, top;
   };
   int32_t top_k = 10;
-  std::vector<Level> levels;
+  std::deque<Level> levels;  // (elements never move: DevBuf is not copyable)
 };
 
 namespace fx {
@@ -296,7 +299,6 @@ extern "C" int fx_attention_build(fx_ctx* ctx, const fx_netlist* net, int32_t nu
     L.nodes = ids;
     L.total.assign(n, 0);
     L.row_ptr.assign((size_t)n + 1, 0);
-    L.top.assign((size_t)n * top_k, -1);
     if (P == 0) continue;
     const int cbits = bits_for(n);
     const int ebits = bits_for(E);
@@ -411,19 +413,21 @@ extern "C" int fx_attention_build(fx_ctx* ctx, const fx_netlist* net, int32_t nu
     }
     FX_HIP(hipGetLastError());
     std::vector<int32_t> rp32((size_t)n + 1);
-    L.col.resize(Zo);
-    L.shared.resize(Zo);
-    L.score.resize(Zo);
     FX_HIP(hipMemcpyAsync(L.total.data(), total.as<int32_t>(), 4 * (size_t)n, hipMemcpyDeviceToHost, st));
     FX_HIP(hipMemcpyAsync(rp32.data(), rowp.as<int32_t>(), 4 * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
+    L.nnz = Zo;
     if (Zo) {
-      FX_HIP(hipMemcpyAsync(L.col.data(), col.as<int32_t>(), 4 * (size_t)Zo, hipMemcpyDeviceToHost, st));
-      FX_HIP(hipMemcpyAsync(L.shared.data(), shr.as<int32_t>(), 4 * (size_t)Zo, hipMemcpyDeviceToHost, st));
-      FX_HIP(hipMemcpyAsync(L.score.data(), score.as<float>(), 4 * (size_t)Zo, hipMemcpyDeviceToHost, st));
+      if ((rc = L.col.alloc(ctx, 4 * (size_t)Zo)) || (rc = L.shared.alloc(ctx, 4 * (size_t)Zo)) ||
+          (rc = L.score.alloc(ctx, 4 * (size_t)Zo)))
+        return rc;
+      FX_HIP(hipMemcpyAsync(L.col.p, col.b.p, 4 * (size_t)Zo, hipMemcpyDeviceToDevice, st));
+      FX_HIP(hipMemcpyAsync(L.shared.p, shr.b.p, 4 * (size_t)Zo, hipMemcpyDeviceToDevice, st));
+      FX_HIP(hipMemcpyAsync(L.score.p, score.b.p, 4 * (size_t)Zo, hipMemcpyDeviceToDevice, st));
     }
-    if (top_k > 0 && Zo > 0)
-      FX_HIP(hipMemcpyAsync(L.top.data(), top.as<int32_t>(), 4 * (size_t)n * top_k,
-                            hipMemcpyDeviceToHost, st));
+    if (top_k > 0 && Zo > 0) {
+      if ((rc = L.top.alloc(ctx, 4 * (size_t)n * top_k))) return rc;
+      FX_HIP(hipMemcpyAsync(L.top.p, top.b.p, 4 * (size_t)n * top_k, hipMemcpyDeviceToDevice, st));
+    }
     FX_HIP(hipStreamSynchronize(st));
     for (int32_t i = 0; i <= n; ++i) L.row_ptr[i] = rp32[i];
   }
@@ -453,7 +457,7 @@ extern "C" int fx_attention_level_info(const fx_attention* a, int32_t level, int
   const auto& L = a->levels[level];
   if (level_index) *level_index = L.depth;
   if (num_clusters) *num_clusters = (int32_t)L.nodes.size();
-  if (nnz) *nnz = (int64_t)L.col.size();
+  if (nnz) *nnz = L.nnz;
   return FX_OK;
 }
 
@@ -470,10 +474,23 @@ extern "C" int fx_attention_level_get(const fx_attention* a, int32_t level, int3
   cp(cluster_node, L.nodes);
   cp(total, L.total);
   cp(row_ptr, L.row_ptr);
-  cp(col, L.col);
-  cp(shared, L.shared);
-  cp(score, L.score);
-  cp(top, L.top);
+  auto dcp = [](void* dst, const fx::DevBuf& b, size_t bytes) -> int {
+    if (!dst || !bytes || !b.p) return FX_OK;
+    FX_HIP(hipMemcpy(dst, b.p, bytes, hipMemcpyDeviceToHost));
+    return FX_OK;
+  };
+  int rc;
+  if ((rc = dcp(col, L.col, 4 * (size_t)L.nnz)) || (rc = dcp(shared, L.shared, 4 * (size_t)L.nnz)) ||
+      (rc = dcp(score, L.score, 4 * (size_t)L.nnz)))
+    return rc;
+  if (top) {
+    const size_t k = (size_t)L.nodes.size() * a->top_k;
+    if (L.top.p) {
+      if ((rc = dcp(top, L.top, 4 * k))) return rc;
+    } else {
+      std::fill(top, top + k, -1);
+    }
+  }
   return FX_OK;
 }
 
