@@ -106,6 +106,8 @@ __device__ double fold_total(const double* part, int m, double* sh, double* out)
   return tree256(q, sh, out);
 }
 
+constexpr int kNbChunk = 8;  // neighbours whose loads are issued together
+
 __global__ __launch_bounds__(kB) void label_kernel(const Seg* __restrict__ segs,
                                                    const int* __restrict__ perm,
                                                    int* __restrict__ pos, int* __restrict__ cl) {
@@ -128,8 +130,19 @@ __global__ __launch_bounds__(kB) void count_kernel(int n, const int* __restrict_
   const int g = perm[p];
   const int L = cl[g];
   int c = 0;
-  if (L >= 0)
-    for (int k = aptr[g]; k < aptr[g + 1]; ++k) c += cl[adj[k]] == L;
+  if (L >= 0) {
+    // Neighbours in chunks: their indices, then their labels, are loaded
+    // together (independent loads in flight instead of two dependent loads
+    // per neighbour).
+    const int k1 = aptr[g + 1];
+    for (int k0 = aptr[g]; k0 < k1; k0 += kNbChunk) {
+      int nb[kNbChunk];
+#pragma unroll
+      for (int q = 0; q < kNbChunk; ++q) nb[q] = k0 + q < k1 ? adj[k0 + q] : -1;
+#pragma unroll
+      for (int q = 0; q < kNbChunk; ++q) c += nb[q] >= 0 && cl[nb[q]] == L;
+    }
+  }
   cnt[p] = c;
 }
 
@@ -146,9 +159,19 @@ __global__ __launch_bounds__(kB) void fill_col_kernel(int n, const int* __restri
   const int L = cl[g];
   if (L < 0) return;
   int o = rp[p];
-  for (int k = aptr[g]; k < aptr[g + 1]; ++k) {
-    const int nb = adj[k];
-    if (cl[nb] == L) col[o++] = pos[nb];
+  const int k1 = aptr[g + 1];
+  for (int k0 = aptr[g]; k0 < k1; k0 += kNbChunk) {
+    int nb[kNbChunk], lb[kNbChunk], ps[kNbChunk];
+#pragma unroll
+    for (int q = 0; q < kNbChunk; ++q) nb[q] = k0 + q < k1 ? adj[k0 + q] : -1;
+#pragma unroll
+    for (int q = 0; q < kNbChunk; ++q) {
+      lb[q] = nb[q] >= 0 ? cl[nb[q]] : -1;
+      ps[q] = nb[q] >= 0 ? pos[nb[q]] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kNbChunk; ++q)  // adjacency order
+      if (nb[q] >= 0 && lb[q] == L) col[o++] = ps[q];
   }
 }
 
