@@ -231,6 +231,24 @@ def test_blif_semantics(tmp_path):
     assert c.net_names == ["x", "y", "q", "z", "a"]
 
 
+def test_circuit_net_names_buffer():
+    # fx_circuit_net_names: size query, exact-size copy, and a short buffer.
+    import ctypes as C
+    from fluxion._lib import check, lib
+    h = C.c_void_p()
+    check(lib.fx_blif_read(os.path.join(GOLDEN, "small.blif").encode(), C.byref(h)))
+    try:
+        need = C.c_int64()
+        assert lib.fx_circuit_net_names(h, None, 0, C.byref(need)) == 0
+        assert need.value == len(b"n1\0n2\0")
+        buf = C.create_string_buffer(need.value)
+        assert lib.fx_circuit_net_names(h, buf, need.value, C.byref(need)) == 0
+        assert buf.raw == b"n1\0n2\0"
+        assert lib.fx_circuit_net_names(h, buf, need.value - 1, C.byref(need)) != 0
+    finally:
+        lib.fx_circuit_free(h)
+
+
 def test_blif_errors(tmp_path):
     with pytest.raises(fxn.FluxionError, match="Failed to open BLIF"):
         fxn.parse_blif(tmp_path / "missing.blif")
