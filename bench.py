@@ -450,7 +450,10 @@ def main():
         job.use_graph(True)
         job.run(1)  # captures the iteration graph
     else:
-        job.set_timing([dominant])
+        # HIP events around the dominant kernel in one timed iteration of
+        # ten (50 launches of the 500): its average launch time for the
+        # roofline, with a tenth of the events' cost to the timed run.
+        job.set_timing([dominant], every=10)
     # The timed run starts from the generated placement: its final HPWL is the
     # state after exactly K iterations (compared with the oracle's run).
     job.set_positions(x0, y0)

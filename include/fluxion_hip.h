@@ -182,6 +182,10 @@ int fx_job_use_graph(fx_job* job, int32_t enable);
  * mask is set, iterations run eagerly (no graph). read_timing returns the
  * summed milliseconds and launch counts per phase ([7] each) and resets. */
 int fx_job_set_timing(fx_job* job, int32_t phase_mask);
+/* Times one iteration in `every` (from the next iteration on; default 1), so
+ * that a long timed run carries few events. No counterpart in the reference
+ * (measurement instrumentation). */
+int fx_job_set_timing_every(fx_job* job, int32_t every);
 int fx_job_read_timing(fx_job* job, double* total_ms, int64_t* count);
 
 /* dispatch_congestion (dispatch.rs:109-180) / compute_congestion_cpu

@@ -435,7 +435,7 @@ static int job_components(const Job& j) {
 }
 
 int phase_begin(Job& j, int phase) {
-  if (!(j.timing_mask & (1 << phase))) return FX_OK;
+  if (!j.timing_now || !(j.timing_mask & (1 << phase))) return FX_OK;
   if (j.ev_used + 2 > j.ev_pool.size()) {
     for (int k = 0; k < 64; ++k) {
       hipEvent_t e;
@@ -450,7 +450,7 @@ int phase_begin(Job& j, int phase) {
 }
 
 int phase_end(Job& j, int phase) {
-  if (!(j.timing_mask & (1 << phase))) return FX_OK;
+  if (!j.timing_now || !(j.timing_mask & (1 << phase))) return FX_OK;
   FX_HIP(hipEventRecord(j.ev_pool[j.ev_used], j.stream));
   j.ev_used += 1;
   return FX_OK;
@@ -460,6 +460,7 @@ constexpr int32_t kForkMinGates = 65536;
 
 static int enqueue_phases(Job& j, int comps, bool integrate) {
   int rc;
+  j.timing_now = (j.timing_iter++ % j.timing_every) == 0;
   // With both, the tree build's first kernel also clears the density counts.
   const bool shared_fill = (comps & 2) && (comps & 4);
   // With both, the net centroids and then, from the key sort on (its deposit
@@ -973,8 +974,19 @@ extern "C" int fx_job_set_timing(fx_job* job, int32_t phase_mask) {
   fx::Job& j = job->j;
   FX_HIP(hipStreamSynchronize(j.stream));
   j.timing_mask = phase_mask & ((1 << fx::kNumPhases) - 1);
+  j.timing_iter = 0;
   j.ev_phase.clear();
   j.ev_used = 0;
+  return FX_OK;
+}
+
+extern "C" int fx_job_set_timing_every(fx_job* job, int32_t every) {
+  fx::clear_error();
+  FX_CHECK(job && every >= 1, FX_E_INVALID, "fx_job_set_timing_every: bad arguments");
+  fx::Job& j = job->j;
+  FX_HIP(hipStreamSynchronize(j.stream));
+  j.timing_every = every;
+  j.timing_iter = 0;
   return FX_OK;
 }
 

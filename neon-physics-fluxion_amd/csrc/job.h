@@ -90,6 +90,9 @@ struct Job {
   // Per-phase HIP-event timing (eager launches only): event pairs recorded
   // on the job stream around the phases selected by timing_mask.
   int timing_mask = 0;
+  int timing_every = 1;       // time one iteration in this many
+  int64_t timing_iter = 0;    // iterations enqueued since timing was set
+  bool timing_now = true;     // the current iteration is timed
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_phase;  // phase of each recorded pair
   size_t ev_used = 0;

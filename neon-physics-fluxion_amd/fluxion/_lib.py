@@ -144,6 +144,7 @@ _SIGS = {
     "fx_job_sync": (C.c_int, [_P]),
     "fx_job_use_graph": (C.c_int, [_P, _I32]),
     "fx_job_set_timing": (C.c_int, [_P, _I32]),
+    "fx_job_set_timing_every": (C.c_int, [_P, _I32]),
     "fx_job_read_timing": (C.c_int, [_P, _P, _P]),
     "fx_congestion": (C.c_int, [_P, _I32, _I32, _P]),
     "fx_hpwl": (C.c_int, [_P, C.POINTER(_D)]),

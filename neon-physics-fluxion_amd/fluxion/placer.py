@@ -390,11 +390,13 @@ class FlatJob:
 
     PHASES = ("springs", "bh_keys", "bh_sort", "bh_tree", "bh_walk", "dct", "combine")
 
-    def set_timing(self, phases=()):
+    def set_timing(self, phases=(), every: int = 1):
+        """Record HIP events around the given phases, in one iteration of `every`."""
         mask = 0
         for ph in phases:
             mask |= 1 << self.PHASES.index(ph)
         check(lib.fx_job_set_timing(self._h, mask))
+        check(lib.fx_job_set_timing_every(self._h, int(every)))
 
     def read_timing(self) -> dict:
         ms = np.zeros(7, np.float64)
