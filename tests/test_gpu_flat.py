@@ -312,3 +312,33 @@ def test_full_size_run_vs_fp64_golden(fast):
           f"overflow gpu {t:.1f} fp64 {g64['overflow_total']:.1f}", flush=True)
     assert abs(h - g64["hpwl"]) / g64["hpwl"] < 5e-3
     assert abs(t - g64["overflow_total"]) / g64["overflow_total"] < 5e-3
+
+
+def test_pooled_buffers_carry_no_state():
+    # Destroyed jobs hand their device blocks to the context's pool; a later job
+    # reuses them with whatever contents they hold. Every run here must equal the
+    # same run on a fresh context, bit for bit (larger job first, so the smaller
+    # ones take over-sized pooled blocks).
+    print("case: pooled buffers reuse", flush=True)
+    shared = fxn.GpuContext.new(0)
+
+    def run(ctx, N, E, seed, G, graph):
+        circ, x, y = fxn.netgen.flat(N, E, seed)
+        with fxn.FlatJob(ctx, circ, fxn.PlacerConfig(density_grid=G, density_weight=DW)) as job:
+            job.set_positions(x, y)
+            job.use_graph(graph)
+            job.run(30)
+            gx, gy = job.positions()
+            return gx, gy, job.hpwl(), job.congestion(64, 64)
+
+    try:
+        run(shared, 40000, 44000, 81, 256, False)
+        for N, seed, G, graph in [(20000, 82, 128, True), (5000, 83, 64, False)]:
+            a = run(shared, N, N + N // 10, seed, G, graph)
+            with fxn.GpuContext.new(0) as fresh:
+                b = run(fresh, N, N + N // 10, seed, G, graph)
+            print(f"N={N}: hpwl pooled {a[2]:.3f} fresh {b[2]:.3f}", flush=True)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+            assert a[2] == b[2] and np.array_equal(a[3], b[3])
+    finally:
+        shared.close()
