@@ -466,38 +466,67 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
     if (lane == 0) masks[wave][d] = m;
   }
   __syncthreads();
-  if (i >= n) return;
   const int D = g->depth;
   const int M = pre[n].c;
   if (i == 0) {
     g->num_nodes = M;
     if ((int64_t)M > cap) g->overflow = 1;
   }
-  const int lp = i > 0 ? (int)L[i - 1] : -1;
-  if (lp >= D) return;  // continuation of a merged leaf
-  const int internal = max(0, min(li, D - 1) - lp);
-  const TreeSum pi = pre[i];
-  const int id0 = pi.c;
+  // Cells starting at position i, shallowest first: internal cells at depths
+  // first_depth.., then the leaf (none for the continuation of a merged
+  // leaf); their ids are pre[i].c, pre[i].c + 1, ...
+  int start = M, cnt = 0, shape = 0;
+  long long pa = 0, pb = 0;
+  if (i < n) {
+    const int lp = i > 0 ? (int)L[i - 1] : -1;
+    const TreeSum pi = pre[i];
+    start = pi.c;
+    if (lp < D) {
+      const int internal = max(0, min(li, D - 1) - lp);
+      const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
+      const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
+      cnt = internal + 1;
+      shape = internal | (leaf_depth << 8) | (first_depth << 16);
+      pa = pi.a;
+      pb = pi.b;
+    }
+  }
+  // The wave's cells are the ids [base, base + total); lane l takes ids
+  // base + l, base + 64 + l, ..., so every lane has work whatever the depth
+  // profile of its 64 positions (a cell's end search is the long part).
+  const int base = __shfl(start, 0, 64);
+  const int total = __shfl(start + cnt, 63, 64) - base;
   const double r0 = g->root[0], r1 = g->root[1], scale = g->scale;
-  const int leaf_depth = (li >= D) ? D : max(lp, li) + 1;
-  const int first_depth = internal > 0 ? lp + 1 : leaf_depth;
-  // Cells starting here, shallowest first: internal cells at depths
-  // first_depth.., then the leaf.
-  for (int k = 0; k <= internal; ++k) {
+  const int pos0 = i - lane;
+  for (int r = 0; r < total; r += 64) {
+    const int id = base + r + lane;
+    // The position owning cell id: the last lane whose first id is <= id
+    // (first ids do not decrease along the wave; a lane with no cells has
+    // the first id of the next, so the last such lane owns cells).
+    int j = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1)
+      if (__shfl(start, j + s, 64) <= id) j += s;
+    const int sj = __shfl(start, j, 64);
+    const int shp = __shfl(shape, j, 64);
+    const long long qa = __shfl(pa, j, 64);
+    const long long qb = __shfl(pb, j, 64);
+    if (r + lane >= total || (int64_t)id >= cap) continue;
+    const int p = pos0 + j;
+    const int k = id - sj;
+    const int internal = shp & 255;
     const bool is_leaf = (k == internal);
-    const int d = is_leaf ? leaf_depth : first_depth + k;
-    const int id = id0 + k;
-    if ((int64_t)id >= cap) continue;
-    const int end = next_lt(L, levels, n, masks, blockIdx.x, i, d);
-    const int count = end - i + 1;
+    const int d = is_leaf ? (shp >> 8) & 255 : (shp >> 16) + k;
+    const int end = next_lt(L, levels, n, masks, blockIdx.x, p, d);
+    const int count = end - p + 1;
     const TreeSum pe = pre[end + 1];
-    const long long sx = pe.a - pi.a;
-    const long long sy = pe.b - pi.b;
+    const long long sx = pe.a - qa;
+    const long long sy = pe.b - qb;
     BhNode nd;
     nd.x = (float)(r0 + ((double)sx / count) / scale);
     nd.y = (float)(r1 + ((double)sy / count) / scale);
     if (is_leaf) {
-      nd.z = (int)vals[i];
+      nd.z = (int)vals[p];
       nd.w = (int)(0x80000000u | (uint32_t)count);
     } else {
       nd.z = pe.c;  // first node after the cell (M after the last cell)
