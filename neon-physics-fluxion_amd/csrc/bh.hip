@@ -189,16 +189,27 @@ __device__ void geometry_from_boxes(const float4* __restrict__ part, int nparts,
   g.size_f = (float)s_root;
 }
 
+// Key bits the radix passes sort (the top 32, four 8-bit digits; runs of
+// equal top halves are ordered afterwards).
+constexpr int kRadixLo = 32, kRadixPasses = 4;
+// Particles per thread of bh_keys.
+constexpr int kKeysPer = 4;
+
 // Quadrant path of each particle (insertIntoChild, BarnesHutTree.cpp:100-134).
 // Every workgroup derives the root cell from the block boxes itself (the
-// same bits in each); workgroup 0 publishes it for the later kernels.
+// same bits in each); workgroup 0 publishes it for the later kernels. Each
+// workgroup also adds its keys' radix digit counts into ghist (the sort's
+// counts, zero between sorts) unless it is null.
 __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict__ x,
                                                   const float* __restrict__ y,
                                                   const float4* __restrict__ part, int nparts,
                                                   BhGeometry* __restrict__ gout,
                                                   uint64_t* __restrict__ keys,
-                                                  uint32_t* __restrict__ vals, OpenRule rule) {
+                                                  uint32_t* __restrict__ vals, OpenRule rule,
+                                                  uint32_t* __restrict__ ghist) {
   __shared__ BhGeometry sg;
+  __shared__ uint32_t hist[kRadixPasses][256];
+  for (int p = 0; p < kRadixPasses; ++p) hist[p][threadIdx.x] = 0;
   if (threadIdx.x < 64) {
     BhGeometry gl;
     geometry_from_boxes(part, nparts, n, gl);
@@ -228,28 +239,53 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
     if (threadIdx.x == 0) gout->open_ok = all_ok;
   }
   const BhGeometry* g = &sg;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double px = (double)x[i], py = (double)y[i];
-  double xl = g->root[0], yl = g->root[1], xh = g->root[2], yh = g->root[3];
   const int D = g->depth;
-  uint64_t key = 0;
-  for (int d = 0; d < D; ++d) {
-    const double mx = (xl + xh) / 2, my = (yl + yh) / 2;
-    uint64_t quad;
-    if (px <= mx) {
-      xh = mx;
-      if (py <= my) { quad = 2; yh = my; }
-      else { quad = 0; yl = my; }
-    } else {
-      xl = mx;
-      if (py <= my) { quad = 3; yh = my; }
-      else { quad = 1; yl = my; }
-    }
-    key |= quad << (62 - 2 * d);
+  // The kKeysPer descents run interleaved (independent FP64 chains), with
+  // selects for the quadrant rule.
+  const int i0 = blockIdx.x * kKeysPer * kBlock + threadIdx.x;
+  double px[kKeysPer], py[kKeysPer], xl[kKeysPer], yl[kKeysPer], xh[kKeysPer], yh[kKeysPer];
+  uint64_t key[kKeysPer];
+#pragma unroll
+  for (int r = 0; r < kKeysPer; ++r) {
+    const int i = i0 + r * kBlock;
+    px[r] = i < n ? (double)x[i] : 0.0;
+    py[r] = i < n ? (double)y[i] : 0.0;
+    xl[r] = g->root[0];
+    yl[r] = g->root[1];
+    xh[r] = g->root[2];
+    yh[r] = g->root[3];
+    key[r] = 0;
   }
-  keys[i] = key;
-  vals[i] = (uint32_t)i;
+  for (int d = 0; d < D; ++d) {
+#pragma unroll
+    for (int r = 0; r < kKeysPer; ++r) {
+      const double mx = (xl[r] + xh[r]) / 2, my = (yl[r] + yh[r]) / 2;
+      const bool west = px[r] <= mx, south = py[r] <= my;
+      xh[r] = west ? mx : xh[r];
+      xl[r] = west ? xl[r] : mx;
+      yh[r] = south ? my : yh[r];
+      yl[r] = south ? yl[r] : my;
+      // NW = 0, NE = 1, SW = 2, SE = 3
+      const uint64_t quad = (west ? 0u : 1u) | (south ? 2u : 0u);
+      key[r] |= quad << (62 - 2 * d);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kKeysPer; ++r) {
+    const int i = i0 + r * kBlock;
+    if (i >= n) continue;
+    keys[i] = key[r];
+    vals[i] = (uint32_t)i;
+    if (ghist) {
+#pragma unroll
+      for (int p = 0; p < kRadixPasses; ++p)
+        atomicAdd(&hist[p][(uint32_t)(key[r] >> (kRadixLo + 8 * p)) & 0xFFu], 1u);
+    }
+  }
+  if (!ghist) return;
+  __syncthreads();
+  for (int p = 0; p < kRadixPasses; ++p)
+    if (hist[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], hist[p][threadIdx.x]);
 }
 
 // After a stable sort on the top 32 key bits: the thread at the head of each
@@ -1005,9 +1041,10 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
                      j.x.as<float>(), j.y.as<float>(), b.part.as<float4>(),
                      b.mhi.as<int32_t>(), (int64_t)5 * b.hi_stride, 127, extra_fill,
                      extra_fill ? extra_fill_n : 0, 0);
-  hipLaunchKernelGGL(bh_keys, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n, j.x.as<float>(),
-                     j.y.as<float>(), (const float4*)b.part.as<float4>(), kReduceBlocks, g,
-                     b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), open_rule(j.p));
+  hipLaunchKernelGGL(bh_keys, dim3(div_up(n, kBlock * kKeysPer)), dim3(kBlock), 0, st, n,
+                     j.x.as<float>(), j.y.as<float>(), (const float4*)b.part.as<float4>(),
+                     kReduceBlocks, g, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(),
+                     open_rule(j.p), n >= 2 ? b.sort.hist.as<uint32_t>() : nullptr);
   FX_HIP(hipGetLastError());
   if ((rc = phase_end(j, kPhaseBhKeys)) || (rc = phase_begin(j, kPhaseBhSort))) return rc;
   // Radix passes over the top 32 key bits only (16 tree levels), then runs
@@ -1017,7 +1054,7 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
   if (n >= 2) {  // (one key is sorted; the sort then sets nothing up)
     SortFinish fin;
     if ((rc = radix_sort_pairs(j.ctx, st, b.sort, b.keys.as<uint64_t>(), b.vals.as<uint32_t>(),
-                               n, 32, 64, &fin)))
+                               n, kRadixLo, kRadixLo + 8 * kRadixPasses, &fin, true)))
       return rc;
     hipLaunchKernelGGL(sort_low_halves, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
                        b.keys.as<uint64_t>(), b.vals.as<uint32_t>(), fin);

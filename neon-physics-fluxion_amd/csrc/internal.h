@@ -104,9 +104,12 @@ struct SortFinish {
 };
 // Stable LSD radix sort over key bits [lo_bit, hi_bit). keys/vals are sorted
 // in place (ping-pong through the workspace). Enqueued on `stream`.
+// `counted`: the caller's kernel already added the keys' digit counts of
+// every pass into ws.hist's first passes x 256 words (8-bit digits from
+// lo_bit up; the words are zero between sorts), so no counting kernel runs.
 int radix_sort_pairs(fx_ctx* ctx, hipStream_t stream, SortWorkspace& ws, uint64_t* keys,
                      uint32_t* vals, int64_t n, int lo_bit, int hi_bit,
-                     SortFinish* finish = nullptr);
+                     SortFinish* finish = nullptr, bool counted = false);
 // Whether a look-back wait of any sort on this workspace timed out (host sync).
 int radix_fault(SortWorkspace& ws, hipStream_t stream, bool* faulted);
 

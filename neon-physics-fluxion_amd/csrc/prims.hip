@@ -209,7 +209,8 @@ int SortWorkspace::ensure(fx_ctx* ctx, hipStream_t st, int64_t n) {
 }
 
 int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* keys,
-                     uint32_t* vals, int64_t n, int lo_bit, int hi_bit, SortFinish* finish) {
+                     uint32_t* vals, int64_t n, int lo_bit, int hi_bit, SortFinish* finish,
+                     bool counted) {
   if (n <= 1) return FX_OK;
   int rc = ws.ensure(ctx, st, n);
   if (rc) return rc;
@@ -228,7 +229,9 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
   // Few blocks, each counting many keys: the per-block flush of 8 x 256
   // counts is the costly part.
   const unsigned hblocks = (unsigned)std::min<int64_t>((n + 2047) / 2048, 256);
-  hipLaunchKernelGGL(radix_histogram, dim3(hblocks), dim3(256), 0, st, ka, n, lo_bit, passes, gbase);
+  if (!counted)
+    hipLaunchKernelGGL(radix_histogram, dim3(hblocks), dim3(256), 0, st, ka, n, lo_bit, passes,
+                       gbase);
   for (int p = 0; p < passes; ++p) {
     uint32_t* cur = (p & 1) ? status1 : status0;
     uint32_t* nxt = (p & 1) ? status0 : status1;
