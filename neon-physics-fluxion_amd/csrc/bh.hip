@@ -285,7 +285,9 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
   if (!ghist) return;
   __syncthreads();
   for (int p = 0; p < kRadixPasses; ++p)
-    if (hist[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], hist[p][threadIdx.x]);
+    if (hist[p][threadIdx.x])
+      atomicAdd(&ghist[(blockIdx.x % kHistCopies) * 2048 + p * 256 + threadIdx.x],
+                hist[p][threadIdx.x]);
 }
 
 // After a stable sort on the top 32 key bits: the thread at the head of each
@@ -300,10 +302,8 @@ __global__ __launch_bounds__(kBlock) void sort_low_halves(int n, uint64_t* __res
                                                           uint32_t* __restrict__ vals,
                                                           SortFinish fin) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (blockIdx.x == 0) {
-    for (int k = threadIdx.x; k < 8 * 256; k += kBlock) fin.ghist[k] = 0;
-    if (threadIdx.x < 8) fin.counters[threadIdx.x] = 0;
-  }
+  for (int k = i; k < kHistWords; k += gridDim.x * kBlock) fin.ghist[k] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 8) fin.counters[threadIdx.x] = 0;
   if (*fin.fault) {
     if (i < n) {
       keys[i] = 0;

@@ -90,7 +90,13 @@ struct SortWorkspace {
   // Allocates for n items and clears the look-back state on `st`.
   int ensure(fx_ctx* ctx, hipStream_t st, int64_t n);
 };
-// The last kernel of a sort clears the digit counts (ghist, 8 x 256) and the
+// Digit counts of a sort: kHistCopies copies of 8 passes x 256 digits. A
+// counting workgroup adds into copy (blockIdx.x % kHistCopies), which puts
+// consecutive workgroups (dispatched round-robin over the XCDs) on
+// different words; a pass sums the copies.
+constexpr int kHistCopies = 8;
+constexpr int kHistWords = kHistCopies * 8 * 256;
+// The last kernel of a sort clears the digit counts (ghist, kHistWords) and the
 // tile counters (8) for the next sort, and after a timed-out look-back
 // (*fault != 0) replaces the output by zero keys and the identity
 // permutation. A caller that passes `finish` gets these pointers and does

@@ -47,7 +47,9 @@ __global__ __launch_bounds__(256) void radix_histogram(const uint64_t* __restric
   }
   __syncthreads();
   for (int p = 0; p < passes; ++p)
-    if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], h[p][threadIdx.x]);
+    if (h[p][threadIdx.x])
+      atomicAdd(&ghist[(blockIdx.x % kHistCopies) * 2048 + p * 256 + threadIdx.x],
+                h[p][threadIdx.x]);
 }
 
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
@@ -69,7 +71,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
   }
   cnt[t] = 0;
   // Global base of digit t: exclusive prefix of this pass's digit counts.
-  const uint32_t c_t = gcount[t];
+  uint32_t c_t = 0;
+#pragma unroll
+  for (int c = 0; c < kHistCopies; ++c) c_t += gcount[c * 2048 + t];
   uint32_t incl = c_t;
   for (int off = 1; off < 64; off <<= 1) {
     const uint32_t u = __shfl_up(incl, off, 64);
@@ -172,10 +176,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
 __global__ void radix_repair(const uint32_t* __restrict__ fault, uint32_t* __restrict__ ghist,
                              uint32_t* __restrict__ counters, uint64_t* __restrict__ keys,
                              uint32_t* __restrict__ vals, int64_t n) {
-  if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) ghist[i] = 0;
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
-  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kHistWords; i += gridDim.x * blockDim.x)
+    ghist[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
   if (*fault == 0) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -188,8 +191,8 @@ int SortWorkspace::ensure(fx_ctx* ctx, hipStream_t st, int64_t n) {
   if (n <= capacity) return FX_OK;
   const int64_t nt = (n + kSortTile - 1) / kSortTile;
   int rc;
-  // hist: 8x256 bases | 2 x ntiles x 256 status words | 8 counters | fault flag
-  const size_t words = 8 * 256 + 2 * (size_t)nt * 256 + 8 + 8;
+  // hist: digit counts | 2 x ntiles x 256 status words | 8 counters | fault flag
+  const size_t words = kHistWords + 2 * (size_t)nt * 256 + 8 + 8;
   if ((rc = keys_alt.alloc(ctx, sizeof(uint64_t) * n)) || (rc = vals_alt.alloc(ctx, sizeof(uint32_t) * n)) ||
       (rc = hist.alloc(ctx, sizeof(uint32_t) * words)))
     return rc;
@@ -218,7 +221,7 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
   const int passes = (hi_bit - lo_bit + 7) / 8;
   if (passes < 1 || passes > 8) return fail(FX_E_INVALID, "radix sort: bad bit range");
   uint32_t* gbase = ws.hist.as<uint32_t>();
-  uint32_t* status0 = gbase + 8 * 256;
+  uint32_t* status0 = gbase + kHistWords;
   uint32_t* status1 = status0 + (size_t)ws.blocks * 256;
   uint32_t* counters = status1 + (size_t)ws.blocks * 256;
   uint32_t* fault = counters + 8;
@@ -270,7 +273,7 @@ int radix_fault(SortWorkspace& ws, hipStream_t st, bool* faulted) {
     *faulted = false;
     return FX_OK;
   }
-  uint32_t* fault = ws.hist.as<uint32_t>() + 8 * 256 + 2 * (size_t)ws.blocks * 256 + 8;
+  uint32_t* fault = ws.hist.as<uint32_t>() + kHistWords + 2 * (size_t)ws.blocks * 256 + 8;
   FX_HIP(hipMemcpyAsync(&f, fault, 4, hipMemcpyDeviceToHost, st));
   FX_HIP(hipStreamSynchronize(st));
   *faulted = f != 0;
