@@ -35,6 +35,7 @@
 #include <cstring>
 
 #include "job.h"
+#include "morton_grid.h"
 #include "run_sort.h"
 #include "walk_open.h"
 
@@ -195,8 +196,9 @@ constexpr int kRadixLo = 32, kRadixPasses = 4;
 // Particles per thread of bh_keys.
 constexpr int kKeysPer = 4;
 
-// Quadrant path of each particle (insertIntoChild, BarnesHutTree.cpp:100-134).
-// Every workgroup derives the root cell from the block boxes itself (the
+// Quadrant path of each particle (insertIntoChild, BarnesHutTree.cpp:100-134;
+// morton_grid.h: the levels whose bisection is exact in FP64 read off a grid,
+// the rest descended). Every workgroup derives the root cell from the block boxes itself (the
 // same bits in each); workgroup 0 publishes it for the later kernels. Each
 // workgroup also adds its keys' radix digit counts into ghist (the sort's
 // counts, zero between sorts) unless it is null.
@@ -208,6 +210,7 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
                                                   uint32_t* __restrict__ vals, OpenRule rule,
                                                   uint32_t* __restrict__ ghist) {
   __shared__ BhGeometry sg;
+  __shared__ KeyGrid kgrid;
   __shared__ uint32_t hist[kRadixPasses][256];
   for (int p = 0; p < kRadixPasses; ++p) hist[p][threadIdx.x] = 0;
   if (threadIdx.x < 64) {
@@ -215,6 +218,7 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
     geometry_from_boxes(part, nparts, n, gl);
     if (threadIdx.x == 0) {
       sg = gl;
+      key_grid_setup(gl.root, gl.depth, &kgrid);
       if (blockIdx.x == 0) {
         gout->root[0] = gl.root[0];
         gout->root[1] = gl.root[1];
@@ -240,46 +244,17 @@ __global__ __launch_bounds__(kBlock) void bh_keys(int n, const float* __restrict
   }
   const BhGeometry* g = &sg;
   const int D = g->depth;
-  // The kKeysPer descents run interleaved (independent FP64 chains), with
-  // selects for the quadrant rule.
-  const int i0 = blockIdx.x * kKeysPer * kBlock + threadIdx.x;
-  double px[kKeysPer], py[kKeysPer], xl[kKeysPer], yl[kKeysPer], xh[kKeysPer], yh[kKeysPer];
-  uint64_t key[kKeysPer];
-#pragma unroll
   for (int r = 0; r < kKeysPer; ++r) {
-    const int i = i0 + r * kBlock;
-    px[r] = i < n ? (double)x[i] : 0.0;
-    py[r] = i < n ? (double)y[i] : 0.0;
-    xl[r] = g->root[0];
-    yl[r] = g->root[1];
-    xh[r] = g->root[2];
-    yh[r] = g->root[3];
-    key[r] = 0;
-  }
-  for (int d = 0; d < D; ++d) {
-#pragma unroll
-    for (int r = 0; r < kKeysPer; ++r) {
-      const double mx = (xl[r] + xh[r]) / 2, my = (yl[r] + yh[r]) / 2;
-      const bool west = px[r] <= mx, south = py[r] <= my;
-      xh[r] = west ? mx : xh[r];
-      xl[r] = west ? xl[r] : mx;
-      yh[r] = south ? my : yh[r];
-      yl[r] = south ? yl[r] : my;
-      // NW = 0, NE = 1, SW = 2, SE = 3
-      const uint64_t quad = (west ? 0u : 1u) | (south ? 2u : 0u);
-      key[r] |= quad << (62 - 2 * d);
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < kKeysPer; ++r) {
-    const int i = i0 + r * kBlock;
-    if (i >= n) continue;
-    keys[i] = key[r];
+    const int i = (blockIdx.x * kKeysPer + r) * kBlock + threadIdx.x;
+    if (i >= n) break;
+    const uint64_t key =
+        D > 0 ? key_grid((double)x[i], (double)y[i], g->root, D, kgrid) : 0ull;
+    keys[i] = key;
     vals[i] = (uint32_t)i;
     if (ghist) {
 #pragma unroll
       for (int p = 0; p < kRadixPasses; ++p)
-        atomicAdd(&hist[p][(uint32_t)(key[r] >> (kRadixLo + 8 * p)) & 0xFFu], 1u);
+        atomicAdd(&hist[p][(uint32_t)(key >> (kRadixLo + 8 * p)) & 0xFFu], 1u);
     }
   }
   if (!ghist) return;
