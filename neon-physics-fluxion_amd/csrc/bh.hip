@@ -507,7 +507,9 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
   // profile of its 64 positions (a cell's end search is the long part).
   const int base = __shfl(start, 0, 64);
   const int total = __shfl(start + cnt, 63, 64) - base;
-  const double r0 = g->root[0], r1 = g->root[1], scale = g->scale;
+  // scale is 2^scale_bits (scale_bits <= 40): multiplying by its exact
+  // reciprocal rounds the same real as dividing by it.
+  const double r0 = g->root[0], r1 = g->root[1], inv_scale = ldexp(1.0, -g->scale_bits);
   const int pos0 = i - lane;
   for (int r = 0; r < total; r += 64) {
     const int id = base + r + lane;
@@ -534,8 +536,8 @@ __global__ __launch_bounds__(kBlock) void bh_emit(
     const long long sx = pe.a - qa;
     const long long sy = pe.b - qb;
     BhNode nd;
-    nd.x = (float)(r0 + ((double)sx / count) / scale);
-    nd.y = (float)(r1 + ((double)sy / count) / scale);
+    nd.x = (float)(r0 + ((double)sx / count) * inv_scale);
+    nd.y = (float)(r1 + ((double)sy / count) * inv_scale);
     if (is_leaf) {
       nd.z = (int)vals[p];
       nd.w = (int)(0x80000000u | (uint32_t)count);
