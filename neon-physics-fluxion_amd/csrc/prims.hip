@@ -3,7 +3,7 @@
 //
 // Radix sort (8-bit digits, "onesweep" structure): one kernel counts the
 // digits of all passes at once; then each pass is ONE kernel over tiles of
-// 1024 items. A tile takes the next tile id from an atomic counter (ids
+// 1,024 items (2,048 from 2^20 keys). A tile takes the next tile id from an atomic counter (ids
 // follow start order, so a tile only ever waits on tiles that are already
 // running), scans the pass's 256 global digit counts into digit bases,
 // counts its own digits, publishes them, and sums its predecessors' counts per digit by
@@ -27,8 +27,13 @@
 namespace fx {
 
 constexpr int kSortThreads = 256;
+// Items per thread of a pass: 4 (tiles of 1,024) below kSortBigN keys, 8
+// (tiles of 2,048) from there on, where fewer, fuller digit runs per tile
+// make the scatter's writes less scattered. The workspace is sized for the
+// 4-item tiles (the larger tile count).
 constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int64_t kSortBigN = 1 << 20;
 constexpr uint32_t kStAgg = 1u << 30, kStInc = 2u << 30, kStMask = (1u << 30) - 1;
 constexpr double kLookBackSeconds = 0.2;
 
@@ -52,6 +57,7 @@ __global__ __launch_bounds__(256) void radix_histogram(const uint64_t* __restric
                 h[p][threadIdx.x]);
 }
 
+template <int kItems>
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, int shift,
     int ntiles, const uint32_t* __restrict__ gcount, uint32_t* __restrict__ status,
@@ -89,11 +95,11 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
     return;
   }
   status_next[(int64_t)tile * 256 + t] = 0;
-  uint64_t k[kSortItems];
-  uint32_t v[kSortItems];
-  const int64_t t0 = (int64_t)tile * kSortTile;
+  uint64_t k[kItems];
+  uint32_t v[kItems];
+  const int64_t t0 = (int64_t)tile * (kItems * kSortThreads);
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < kItems; ++r) {
     const int64_t i = t0 + r * kSortThreads + t;
     k[r] = 0;
     v[r] = 0;
@@ -138,7 +144,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep(
   const bool write_ok = timed_out == 0;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < kItems; ++r) {
     const int64_t i = t0 + r * kSortThreads + t;
     const bool valid = i < n;
     const uint32_t d = (uint32_t)(k[r] >> shift) & 0xFFu;
@@ -217,7 +223,9 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
   if (n <= 1) return FX_OK;
   int rc = ws.ensure(ctx, st, n);
   if (rc) return rc;
-  const int nt = (int)((n + kSortTile - 1) / kSortTile);
+  const bool big = n >= kSortBigN;
+  const int64_t tile_items = (big ? 8 : 4) * kSortThreads;
+  const int nt = (int)((n + tile_items - 1) / tile_items);
   const int passes = (hi_bit - lo_bit + 7) / 8;
   if (passes < 1 || passes > 8) return fail(FX_E_INVALID, "radix sort: bad bit range");
   uint32_t* gbase = ws.hist.as<uint32_t>();
@@ -238,9 +246,14 @@ int radix_sort_pairs(fx_ctx* ctx, hipStream_t st, SortWorkspace& ws, uint64_t* k
   for (int p = 0; p < passes; ++p) {
     uint32_t* cur = (p & 1) ? status1 : status0;
     uint32_t* nxt = (p & 1) ? status0 : status1;
-    hipLaunchKernelGGL(radix_onesweep, dim3(nt), dim3(kSortThreads), 0, st, ka, va, n,
-                       lo_bit + 8 * p, nt, (const uint32_t*)(gbase + p * 256), cur, nxt,
-                       counters + p, fault, ws.spin_ticks, kb, vb);
+    if (big)
+      hipLaunchKernelGGL(radix_onesweep<8>, dim3(nt), dim3(kSortThreads), 0, st, ka, va, n,
+                         lo_bit + 8 * p, nt, (const uint32_t*)(gbase + p * 256), cur, nxt,
+                         counters + p, fault, ws.spin_ticks, kb, vb);
+    else
+      hipLaunchKernelGGL(radix_onesweep<4>, dim3(nt), dim3(kSortThreads), 0, st, ka, va, n,
+                         lo_bit + 8 * p, nt, (const uint32_t*)(gbase + p * 256), cur, nxt,
+                         counters + p, fault, ws.spin_ticks, kb, vb);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
