@@ -977,7 +977,8 @@ int bh_alloc(Job& j) {
       (rc = fill_i32(j.stream, b.m2.as<int32_t>(), pad16(b.nb1) / 4, 0x7F7F7F7F)))
     return rc;
   // Lanes per particle in the walk. Measured on MI355X over the 500-iteration
-  // 200k-gate run: W=1 0.185 ms, W=2 0.206 ms, W=4 0.356 ms per walk.
+  // 200k-gate run: W=1 0.225 ms, W=2 0.339 ms per walk (W=1 walks the whole
+  // pre-order range in one loop; W>1 splits it at the root's children).
   b.walk_lanes = 1;
   if (const char* e = getenv("FLUXION_WALK_LANES")) {
     const int w = atoi(e);
