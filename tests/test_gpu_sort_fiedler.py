@@ -8,7 +8,7 @@ from oracle import oracle as orc
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [1, 7, 2048, 2049, 100000, 1000003, 1048593, 3000017])
+@pytest.mark.parametrize("n", [1, 7, 2048, 2049, 100000, 1000003, 1048593, 3000017, 4194319])
 def test_sort_u64_stable(ctx, n):
     rng = np.random.default_rng(n)
     k = rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True)
