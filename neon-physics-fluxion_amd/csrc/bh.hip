@@ -89,15 +89,28 @@ __global__ __launch_bounds__(kBlock) void bh_bbox_partial(int n, const float* __
   }
   float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
   float nz = INFINITY, bad = 0.0f;  // smallest non-zero magnitude; 1 if a NaN was seen
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float xi = x[i], yi = y[i];
-    mnx = fminf(mnx, xi);
-    mxx = fmaxf(mxx, xi);
-    mny = fminf(mny, yi);
-    mxy = fmaxf(mxy, yi);
-    if (xi != 0.0f) nz = fminf(nz, fabsf(xi));
-    if (yi != 0.0f) nz = fminf(nz, fabsf(yi));
-    if (xi != xi || yi != yi) bad = 1.0f;
+  // Four strided positions per round, their loads issued together (minima
+  // and maxima do not depend on the order).
+  const int stride = gridDim.x * blockDim.x;
+  for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    float xs[4], ys[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * stride;
+      xs[u] = i < n ? x[i] : x[i0];
+      ys[u] = i < n ? y[i] : y[i0];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float xi = xs[u], yi = ys[u];
+      mnx = fminf(mnx, xi);
+      mxx = fmaxf(mxx, xi);
+      mny = fminf(mny, yi);
+      mxy = fmaxf(mxy, yi);
+      if (xi != 0.0f) nz = fminf(nz, fabsf(xi));
+      if (yi != 0.0f) nz = fminf(nz, fabsf(yi));
+      if (xi != xi || yi != yi) bad = 1.0f;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     mnx = fminf(mnx, __shfl_down(mnx, off, 64));
