@@ -370,12 +370,19 @@ __global__ __launch_bounds__(kBlock) void bh_lcp_count(
     TreeSum tot = wtot[0];
     for (int w = 1; w < kBlock / 64; ++w) tot = ts_add(tot, wtot[w]);
     tsum[b] = tot;
+    // Group totals (16 blocks each; integer atomics, so exact in any order),
+    // zero on entry: the emit kernel of the previous build cleared them.
+    TreeSum* gs = tsum + gridDim.x + 1 + (b >> 4);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&gs->a), (unsigned long long)tot.a);
+    atomicAdd(reinterpret_cast<unsigned long long*>(&gs->b), (unsigned long long)tot.b);
+    atomicAdd(&gs->c, tot.c);
   }
 }
 
 // pre[i] = sum of tri[0..i), pre[n] = the total: block prefix + in-block scan.
-// Every tile adds up the totals of the tiles before it itself (integer sums,
-// so the order does not matter), which saves a launch for scanning them.
+// Every tile adds up the totals before it itself — whole groups of 16 tiles,
+// then the tiles of its own group before it (integer sums, so the order does
+// not matter) — which saves a launch for scanning them.
 __global__ __launch_bounds__(kBlock) void tree_scan_tiles(int n, const TreeSum* __restrict__ tri,
                                                           const TreeSum* __restrict__ tsum,
                                                           TreeSum* __restrict__ pre) {
@@ -384,7 +391,10 @@ __global__ __launch_bounds__(kBlock) void tree_scan_tiles(int n, const TreeSum* 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   TreeSum before = {0, 0, 0, 0};
-  for (int k = threadIdx.x; k < (int)blockIdx.x; k += kBlock) before = ts_add(before, tsum[k]);
+  const int b = blockIdx.x, g0 = b & ~15;
+  const TreeSum* gsum = tsum + gridDim.x + 1;
+  for (int k = threadIdx.x; k < (b >> 4); k += kBlock) before = ts_add(before, gsum[k]);
+  if (threadIdx.x < b - g0) before = ts_add(before, tsum[g0 + threadIdx.x]);
   for (int off = 32; off > 0; off >>= 1) {
     before.a += __shfl_down(before.a, off, 64);
     before.b += __shfl_down(before.b, off, 64);
@@ -480,8 +490,12 @@ __device__ __forceinline__ int next_lt(const int8_t* __restrict__ L, const MinLe
 __global__ __launch_bounds__(kBlock) void bh_emit(
     int n, const uint32_t* __restrict__ vals, const int8_t* __restrict__ L, MinLevels levels,
     const TreeSum* __restrict__ pre,
-    BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap) {
+    BhGeometry* __restrict__ g, BhNode* __restrict__ nodes, int64_t cap,
+    TreeSum* __restrict__ gsum, int ngroups) {
   __shared__ unsigned long long masks[kBlock / 64][33];
+  // The tree scan has read the group totals: clear them for the next build.
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k < ngroups; k += kBlock) gsum[k] = TreeSum{0, 0, 0, 0};
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = i < n ? (int)L[i] : -1;
@@ -980,11 +994,13 @@ int bh_alloc(Job& j) {
       (rc = b.m1.alloc(c, pad16(c1))) || (rc = b.m2.alloc(c, pad16(b.nb1))) ||
       (rc = b.mhi.alloc(c, 4 * 5 * b.hi_stride)) ||
       (rc = b.tri.alloc(c, sizeof(TreeSum) * n)) ||
-      (rc = b.tsum.alloc(c, sizeof(TreeSum) * (b.nb1 + 1))) ||
+      (rc = b.tsum.alloc(c, sizeof(TreeSum) * (b.nb1 + 1 + (b.nb1 + 15) / 16))) ||
       (rc = b.pre.alloc(c, sizeof(TreeSum) * (n + 1))) ||
       (rc = b.fx.alloc(c, 4 * n)) || (rc = b.fy.alloc(c, 4 * n)) ||
       (rc = b.stats.alloc(c, 64)) || (rc = b.sort.ensure(c, j.stream, n)) ||
       (rc = fill_i32(j.stream, b.geom.as<int32_t>(), sizeof(BhGeometry) / 4, 0)) ||
+      (rc = fill_i32(j.stream, b.tsum.as<int32_t>(),
+                     (int64_t)(sizeof(TreeSum) / 4) * (b.nb1 + 1 + (b.nb1 + 15) / 16), 0)) ||
       (rc = fill_i32(j.stream, b.lcp.as<int32_t>(), pad16(n) / 4, 0x7F7F7F7F)) ||
       (rc = fill_i32(j.stream, b.m1.as<int32_t>(), pad16(c1) / 4, 0x7F7F7F7F)) ||
       (rc = fill_i32(j.stream, b.m2.as<int32_t>(), pad16(b.nb1) / 4, 0x7F7F7F7F)))
@@ -1073,7 +1089,8 @@ static int bh_build(Job& j, int32_t* extra_fill = nullptr, int64_t extra_fill_n 
   lv.hi_stride = (int)b.hi_stride;
   hipLaunchKernelGGL(bh_emit, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, n,
                      b.vals.as<uint32_t>(), b.lcp.as<int8_t>(), lv,
-                     (const TreeSum*)b.pre.as<TreeSum>(), g, b.nodes.as<BhNode>(), b.node_cap);
+                     (const TreeSum*)b.pre.as<TreeSum>(), g, b.nodes.as<BhNode>(), b.node_cap,
+                     b.tsum.as<TreeSum>() + b.nb1 + 1, (int)((b.nb1 + 15) / 16));
   FX_HIP(hipGetLastError());
   return phase_end(j, kPhaseBhTree);
 }
